@@ -1,0 +1,181 @@
+"""Throughput of the Fourier-domain DAS beamformer on one MI355X.
+
+Workload: plain CPWC at full size -- 128 elements, 75 steered plane waves
+(-18.5..18.5 deg, 0.5 deg steps), 3328 samples, 1-9 MHz band, 1024 x 512
+voxels -- with the frequency-dependent F-number grating.angle_lb(60, 1.5) and
+a Hann receive window (headline), plus the fixed F = 1.5 boxcar arm.  One step
+is one compounded frame: band-limited spectra of all 9600 channels plus the
+fused beamforming kernel, from device-resident float32 RF to a device-resident
+complex64 image.  Synthetic seeded data (the reference ships none).
+
+Prints one JSON line: Mvoxel*frames/s, the roofline of the dominant kernel
+(beamforming, MFMA-bound, FP32 peak) from HIP events recorded inside the
+library on the launch stream, and the reference CPU path (FP64 C restatement
+of das_pw.m, oracle/) timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--arm", choices=["both", "hann", "boxcar"], default="both")
+    ap.add_argument("--workload", choices=["cpwc_full", "cpwc_small", "saft_full"], default="cpwc_full")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, window_name, F_kind, F_params, n_threads):
+    """Reference CPU path (FP64 C restatement of das_pw.m:302-485) on a bounded sample:
+    all spectra + one lateral column of voxels, extrapolated linearly in voxels."""
+    from oracle import oracle as O
+
+    O.build()
+    rf = w.rf_host()
+    F = O.fnumber(F_kind, *F_params)
+    ix_col = w.positions_x.size // 2
+    sel = (np.full(w.positions_z.size, ix_col), np.arange(w.positions_z.size))
+    # spectra alone (the FFT of das_pw.m:383)
+    N, lb, ub = O.grid(w.n_samples, w.n_elements, w.pitch, w.positions_z[0], w.f_s, w.c_0, *w.f_bounds)
+    t0 = time.perf_counter()
+    O.spectra(rf.reshape(-1, w.n_samples), N, lb, ub, mode=1, n_threads=n_threads)
+    t_spec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.das_pw(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0, w.f_bounds, 0,
+             (window_name, 0.0), F, "on", n_threads=n_threads, sel=sel)
+    t_col = time.perf_counter() - t0 - t_spec  # das_pw recomputes the spectra
+    t_frame = t_spec + max(t_col, 0.0) * w.positions_x.size
+    return {
+        "value": w.n_voxels / t_frame / 1e6,
+        "unit": "Mvoxel*frames/s",
+        "cores": n_threads,
+        "kind": "port",
+        "sample": f"spectra of all {w.n_tx * w.n_elements} channels + 1 of {w.positions_x.size} lateral "
+                  f"columns ({w.positions_z.size} voxels, all {w.n_tx} angles), extrapolated x{w.positions_x.size}; "
+                  f"sample wall {t_spec + t_col:.1f} s",
+    }
+
+
+def run_arm(w, plan_kw, steps, warmup, dev):
+    import torch
+
+    import f_number_amd as fa
+
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     w.c_0, w.f_bounds, 0.0, index_gpu=dev.index or 0, **plan_kw)
+    info = plan.info()
+    rf = w.rf_device(dev)
+    img = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(warmup):
+        plan.execute(rf, img, stream=stream)
+    torch.cuda.synchronize(dev)
+    plan.set_profiling(True)
+    plan.get_profile()  # reset
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute(rf, img, stream=stream)
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    spec_ms, beam_ms, n = plan.get_profile()
+    plan.set_profiling(False)
+    ok = bool(torch.isfinite(torch.view_as_real(img)).all().item())
+    plan.close()
+    return {
+        "ms_per_step": wall / steps * 1e3,
+        "spectra_ms": spec_ms / max(n, 1),
+        "beamform_ms": beam_ms / max(n, 1),
+        "info": info,
+        "finite": ok,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
+                                   "(sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+
+    import f_number_amd as fa
+    from f_number_amd import synth
+
+    dev = torch.device("cuda:0")
+    w = {"cpwc_full": synth.cpwc_full, "cpwc_small": synth.cpwc_small, "saft_full": synth.saft_full}[args.workload]()
+    arms = {}
+    if args.arm in ("both", "hann"):
+        arms["hann_angle_lb"] = run_arm(w, dict(window=fa.windows.hann(),
+                                                F_number=fa.f_numbers.grating.angle_lb(60, 1.5)),
+                                        args.steps, args.warmup, dev)
+    if args.arm in ("both", "boxcar"):
+        arms["boxcar_F1.5"] = run_arm(w, dict(window=fa.windows.boxcar(), F_number=fa.f_numbers.constant(1.5)),
+                                      args.steps, args.warmup, dev)
+    head_key = "hann_angle_lb" if "hann_angle_lb" in arms else "boxcar_F1.5"
+    head = arms[head_key]
+    value = w.n_voxels / (head["ms_per_step"] / 1e3) / 1e6
+    flops = head["info"]["flops_per_frame"]
+    achieved = flops / (head["beamform_ms"] / 1e3) / 1e12
+    out = {
+        "metric": "Mvoxel*frames/s",
+        "value": value,
+        "unit": "Mvoxel*frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded point + speckle scatterers, 5 MHz pulse, 30 dB SNR)",
+        "config": {
+            "workload": w.name,
+            "arm": "grating.angle_lb(60,1.5) F-number, Hann receive window, normalization on"
+            if head_key == "hann_angle_lb" else "constant F=1.5, boxcar",
+            "n_elements": w.n_elements, "n_angles": int(w.n_tx), "n_samples": w.n_samples,
+            "grid": [int(w.positions_z.size), int(w.positions_x.size)],
+            "f_bounds_hz": list(w.f_bounds), "N_dft": head["info"]["N_dft"], "n_f": head["info"]["n_f"],
+            "precision": "fp32", "semantics": "cpu",
+        },
+        "roofline": {
+            "kernel": "beamform_kernel",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "flops_per_launch": flops,
+            "avg_launch_ms": head["beamform_ms"],
+        },
+        "stages_ms": {"spectra": head["spectra_ms"], "beamform": head["beamform_ms"]},
+        "arms": {k: {"value": w.n_voxels / (v["ms_per_step"] / 1e3) / 1e6, "ms_per_step": v["ms_per_step"],
+                     "beamform_ms": v["beamform_ms"], "spectra_ms": v["spectra_ms"], "finite": v["finite"]}
+                 for k, v in arms.items()},
+    }
+    if not args.no_cpu_baseline:
+        nthreads = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(w, "hann", "angle_lb", (60, 1.5, 10), nthreads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

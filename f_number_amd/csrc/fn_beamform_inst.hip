@@ -1,0 +1,84 @@
+// fn_beamform_inst.hip -- explicit instantiations of beamform_kernel for one
+// (mode, KI) pair, compiled once per pair (-DINST_MODE=0|1 -DINST_KI=8|16|32)
+// so the instantiation set builds in parallel.
+#include "fn_kernels.hip.h"
+
+#ifndef INST_MODE
+#error "INST_MODE must be defined"
+#endif
+#ifndef INST_KI
+#error "INST_KI must be defined"
+#endif
+
+#define FN_CAT4(a, b, c, d) a##b##c##d
+#define FN_LAUNCHER(m, k) FN_CAT4(launch_beamform_m, m, _k, k)
+
+namespace fnk {
+
+template <int MT, int WCR, int WCT>
+static hipError_t launch_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    auto kern = beamform_kernel<INST_MODE, MT, INST_KI, WCR, WCT>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
+    return hipGetLastError();
+}
+
+#if INST_MODE == 0
+// plane waves: any number of transmit M-tiles per group (1..8), receive window class
+template <int MT>
+static hipError_t launch_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    switch (wcr) {
+    case WC_RANGE: return launch_one<MT, WC_RANGE, WC_RANGE>(grid, lds, s, A);
+    case WC_LINEAR: return launch_one<MT, WC_LINEAR, WC_RANGE>(grid, lds, s, A);
+    default: return launch_one<MT, WC_COS, WC_RANGE>(grid, lds, s, A);
+    }
+}
+
+hipError_t FN_LAUNCHER(INST_MODE, INST_KI)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
+                                            const BeamArgs &A)
+{
+    (void)wct;
+    switch (mt) {
+    case 1: return launch_mt<1>(wcr, grid, lds, s, A);
+    case 2: return launch_mt<2>(wcr, grid, lds, s, A);
+    case 3: return launch_mt<3>(wcr, grid, lds, s, A);
+    case 4: return launch_mt<4>(wcr, grid, lds, s, A);
+    case 5: return launch_mt<5>(wcr, grid, lds, s, A);
+    case 6: return launch_mt<6>(wcr, grid, lds, s, A);
+    case 7: return launch_mt<7>(wcr, grid, lds, s, A);
+    case 8: return launch_mt<8>(wcr, grid, lds, s, A);
+    default: return hipErrorInvalidValue;
+    }
+}
+#else
+// synthetic aperture: transmit rows = elements, MT = 4 KI / 16 = KI / 4 tiles
+constexpr int SA_MT = INST_KI / 4;
+template <int WCR>
+static hipError_t launch_wct(int wct, dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    switch (wct) {
+    case WC_RANGE: return launch_one<SA_MT, WCR, WC_RANGE>(grid, lds, s, A);
+    case WC_LINEAR: return launch_one<SA_MT, WCR, WC_LINEAR>(grid, lds, s, A);
+    default: return launch_one<SA_MT, WCR, WC_COS>(grid, lds, s, A);
+    }
+}
+
+hipError_t FN_LAUNCHER(INST_MODE, INST_KI)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
+                                            const BeamArgs &A)
+{
+    if (mt != SA_MT) return hipErrorInvalidValue;
+    switch (wcr) {
+    case WC_RANGE: return launch_wct<WC_RANGE>(wct, grid, lds, s, A);
+    case WC_LINEAR: return launch_wct<WC_LINEAR>(wct, grid, lds, s, A);
+    default: return launch_wct<WC_COS>(wct, grid, lds, s, A);
+    }
+}
+#endif
+
+}  // namespace fnk

@@ -1,0 +1,552 @@
+// fn_core.hip -- plans, kernel dispatch and the GPU entry points of the C ABI
+// (fn_das_pw / fn_das_sa / fn_plan_*).  Replaces the orchestration of
+// gpu_bf_das_pw_rf.cu:700-1122 and gpu_bf_saft_rf.cu:800-1292: device
+// buffers, spectra, per-frequency beamforming and the output scale.
+#include "fn_internal.h"
+#include "fn_kernels.hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace fnk {
+hipError_t launch_beamform_m0_k8(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_m0_k16(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_m0_k32(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_m1_k8(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_m1_k16(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_m1_k32(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+// zero fill used instead of hipMemsetAsync so the launch sequence stays
+// capturable into a graph
+__global__ void fill_zero_kernel(float *p, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+}  // namespace fnk
+
+using fnh::fail;
+
+namespace {
+
+constexpr size_t kMaxLds = 160 * 1024;  // gfx950 LDS per workgroup
+
+struct Run {
+    hipEvent_t e0, e1, e2;
+};
+
+}  // namespace
+
+struct fn_plan {
+    int mode = 0;  // 0: plane waves, 1: synthetic aperture
+    int device = 0;
+    fnh::Grid g;
+    int n_x = 0, n_z = 0, n_t = 0, n_el = 0, n_tx = 0;
+    long n_vox = 0;
+    int KI = 0, MT_all = 0, MT_tmpl = 0, n_groups = 1, bin_split = 1;
+    int wcr = 0, wct = 0;
+    int verbosity = 0;
+    fnk::BeamArgs args{};
+    fnk::SpecArgs sargs{};
+    size_t lds_beam = 0, lds_spec = 0;
+    double *d_px = nullptr, *d_pz = nullptr, *d_Frx = nullptr, *d_Ftx = nullptr, *d_txp = nullptr;
+    float2 *d_tw = nullptr, *d_X = nullptr;
+    size_t x_bytes = 0;
+    hipStream_t stream = nullptr;
+    int profiling = 0;
+    std::vector<Run> pending, spare;
+    double spec_ms = 0.0, beam_ms = 0.0;
+    int runs = 0;
+    double flops = 0.0, useful_flops = 0.0;
+};
+
+namespace {
+
+int hip_fail(hipError_t e, const char *what)
+{
+    if (e == hipErrorOutOfMemory) return fail(FN_ERR_OUT_OF_MEMORY, "%s: out of device memory", what);
+    return fail(FN_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_CHECK(call, what)                          \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+int select_device(int index)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1) return fail(FN_ERR_NO_DEVICES, "Could not find any HIP devices!");
+    // gpu_bf_das_pw_rf.cu:460 tests (index >= 0 || index < count), which always
+    // passes; the intended check is applied here.
+    if (index < 0 || index >= count) return fail(FN_ERR_DEVICE, "Selected device does not exist!");
+    HIP_CHECK(hipSetDevice(index), "hipSetDevice");
+    return FN_OK;
+}
+
+// window -> (class, coefficients, support); see fn_kernels.hip.h for the forms
+int setup_window(const fn_window &w, int semantics, fnk::WinCoef *c, int *wc, int *support)
+{
+    const bool cpu = semantics == FN_SEMANTICS_CPU;
+    *c = fnk::WinCoef{-1.f, INFINITY, 0.f, 1.f, 0.f, 0.f, 0.f};
+    switch (w.id) {
+    case FN_WINDOW_BOXCAR:
+        *wc = fnk::WC_RANGE;
+        *support = cpu ? fnk::SUPPORT_OPEN_H : fnk::SUPPORT_SNAPPED;
+        return FN_OK;
+    case FN_WINDOW_TUKEY:
+        *wc = fnk::WC_COS;
+        c->t_flat = c->t_off = (float)(1.0 - w.parameter);
+        c->t_cut = 1.f;
+        c->A = c->B = 0.5f;
+        c->s_half = (float)(0.5 / w.parameter);
+        *support = fnk::SUPPORT_SNAPPED;  // the window is zero beyond the snapped aperture
+        return FN_OK;
+    case FN_WINDOW_HANN:
+        *wc = fnk::WC_COS;
+        c->t_cut = cpu ? INFINITY : 1.f;
+        c->A = c->B = 0.5f;
+        c->s_half = 0.5f;
+        *support = cpu ? fnk::SUPPORT_ALL : fnk::SUPPORT_SNAPPED;
+        return FN_OK;
+    case FN_WINDOW_TRIANGULAR:
+        *wc = fnk::WC_LINEAR;
+        c->t_cut = cpu ? INFINITY : 1.f;
+        c->A = 1.f;
+        c->C = -1.f;
+        *support = cpu ? fnk::SUPPORT_ALL : fnk::SUPPORT_SNAPPED;
+        return FN_OK;
+    default:
+        // id 3 (cosine) has no windows.* class (gpu_bf_helper_functions.cu:56-97 never yields it)
+        return fail(FN_ERR_WINDOW, "window id %d is not reachable from any windows.* class", (int)w.id);
+    }
+}
+
+int ki_of(int n_el)
+{
+    if (n_el <= 32) return 8;
+    if (n_el <= 64) return 16;
+    if (n_el <= 128) return 32;
+    return 0;
+}
+
+struct Common {
+    const double *px, *pz;
+    int n_x, n_z, n_t, n_el, n_tx;
+    double f_s, width, pitch, c0, t0;
+    int semantics, precision, scale, index_gpu, verbosity;
+};
+
+void destroy(fn_plan *p)
+{
+    if (!p) return;
+    hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp);
+    hipFree(p->d_tw); hipFree(p->d_X);
+    for (auto &r : p->pending) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
+    for (auto &r : p->spare) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
+    if (p->stream) hipStreamDestroy(p->stream);
+    delete p;
+}
+
+template <typename T>
+int upload(T **dst, const T *src, size_t n, const char *what)
+{
+    HIP_CHECK(hipMalloc(dst, sizeof(T) * n), what);
+    HIP_CHECK(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice), what);
+    return FN_OK;
+}
+
+// shared part of plan creation
+int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double *Frx, const double *Ftx)
+{
+    if (c.precision != FN_PRECISION_FP32)
+        return fail(FN_ERR_UNSUPPORTED, "precision %d is not available in this build (FP32 only)", c.precision);
+    int s = select_device(c.index_gpu);
+    if (s) return s;
+    P->device = c.index_gpu;
+    P->g = g;
+    P->n_x = c.n_x; P->n_z = c.n_z; P->n_t = c.n_t; P->n_el = c.n_el; P->n_tx = c.n_tx;
+    P->n_vox = (long)c.n_x * c.n_z;
+    P->verbosity = c.verbosity;
+    P->KI = ki_of(c.n_el);
+    if (!P->KI) return fail(FN_ERR_UNSUPPORTED, "at most 128 array elements are supported (got %d)", c.n_el);
+    P->MT_all = (c.n_tx + 15) / 16;
+    if (P->mode == 0) {
+        const int groups = (P->MT_all + 7) / 8;
+        P->MT_tmpl = (P->MT_all + groups - 1) / groups;
+        P->n_groups = (P->MT_all + P->MT_tmpl - 1) / P->MT_tmpl;
+    } else {
+        P->MT_tmpl = P->KI / 4;
+        P->n_groups = 1;
+    }
+    P->lds_beam = (size_t)P->MT_tmpl * P->KI * 64 * sizeof(float2);
+    P->lds_spec = (size_t)g.N * sizeof(float2) + 64 * 65 * sizeof(float);
+    if (P->lds_beam > kMaxLds) return fail(FN_ERR_UNSUPPORTED, "transmit tile exceeds LDS");
+    if (P->lds_spec > kMaxLds) return fail(FN_ERR_UNSUPPORTED, "DFT order %d too large for the LDS twiddle table", g.N);
+
+    // bin split: enough workgroups to fill 256 CUs several times over
+    const long tiles = (P->n_vox + 63) / 64;
+    const long blocks = tiles * P->n_groups;
+    int split = 1;
+    if (blocks < 2048) split = (int)std::min<long>((2048 + blocks - 1) / blocks, std::max(1, g.n_f / 8));
+    const int chunk = (g.n_f + split - 1) / split;
+    P->bin_split = (g.n_f + chunk - 1) / chunk;
+
+    HIP_CHECK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if ((s = upload(&P->d_px, c.px, c.n_x, "positions_x"))) return s;
+    if ((s = upload(&P->d_pz, c.pz, c.n_z, "positions_z"))) return s;
+    if ((s = upload(&P->d_Frx, Frx, g.n_f, "F-number"))) return s;
+    if (Ftx && (s = upload(&P->d_Ftx, Ftx, g.n_f, "F-number tx"))) return s;
+    // twiddles e^{+j 2 pi m / N} rounded once from long double (conjugated in the kernel)
+    std::vector<float2> tw(g.N);
+    for (int m = 0; m < g.N; ++m) {
+        const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)g.N;
+        tw[m] = make_float2((float)cosl(ang), (float)sinl(ang));
+    }
+    if ((s = upload(&P->d_tw, tw.data(), tw.size(), "twiddles"))) return s;
+    P->x_bytes = (size_t)g.n_f * P->MT_all * P->KI * 64 * sizeof(float2);
+    HIP_CHECK(hipMalloc(&P->d_X, P->x_bytes), "spectra workspace");
+    HIP_CHECK(hipMemset(P->d_X, 0, P->x_bytes), "spectra workspace");  // padding rows stay zero
+
+    fnk::BeamArgs &A = P->args;
+    A.X = P->d_X;
+    A.px = P->d_px; A.pz = P->d_pz; A.Frx = P->d_Frx; A.Ftx = P->d_Ftx; A.txp = P->d_txp;
+    A.n_x = c.n_x; A.n_z = c.n_z; A.n_el = c.n_el; A.n_tx = c.n_tx;
+    A.n_f = g.n_f; A.n_lb = g.n_lb; A.N = g.N;
+    A.bin_chunk = chunk;
+    A.tx_group = P->MT_tmpl;
+    A.atomic_out = (P->bin_split > 1 || P->n_groups > 1) ? 1 : 0;
+    A.M = (c.n_el - 1) / 2.0;
+    A.pitch = c.pitch; A.width = c.width; A.c0 = c.c0; A.f_s = c.f_s; A.t0 = c.t0;
+    // das_pw.m:384,467 (CPU: 2 * sum) vs gpu_bf_das_pw_rf.cu:1118-1119 (MEX: 2 / N * sum)
+    A.out_scale = (c.scale == FN_SCALE_CPU) ? 2.0f : (float)(2.0 / g.N);
+
+    fnk::SpecArgs &S = P->sargs;
+    S.n_t = c.n_t; S.n_ch = c.n_el * c.n_tx; S.N = g.N; S.n_lb = g.n_lb; S.n_f = g.n_f;
+    S.n_el = c.n_el; S.n_tx = c.n_tx; S.MT = P->MT_all; S.KI = P->KI;
+
+    P->flops = 8.0 * g.n_f * (double)P->n_vox * c.n_el * c.n_tx;
+    return FN_OK;
+}
+
+// useful (valid voxel, bin) count of the receive aperture, for the roofline
+double useful_fraction(const fnh::Grid &g, const double *px, int nx, const double *pz, int nz, const double *F,
+                       double f_s, double c0, double pitch, int n_el, bool clamp)
+{
+    const double M = (n_el - 1) / 2.0;
+    long valid = 0, total = 0;
+    const int sx = std::max(1, nx / 64), sz = std::max(1, nz / 64);
+    for (int ix = 0; ix < nx; ix += sx)
+        for (int iz = 0; iz < nz; iz += sz)
+            for (int b = 0; b < g.n_f; ++b) {
+                double Fv = F[b];
+                const double fn = f_s * (double)(g.n_lb + b) / g.N;
+                if (clamp) Fv = std::fmin(Fv, std::sqrt(fn * pz[iz] / (2.88 * c0)));
+                const double h = pz[iz] / (2.0 * Fv);
+                const double lo = std::fmax(std::ceil(M + (px[ix] - h) / pitch), 0.0);
+                const double hi = std::fmin(std::floor(M + (px[ix] + h) / pitch), n_el - 1.0);
+                valid += lo <= hi;
+                ++total;
+            }
+    return total ? (double)valid / total : 0.0;
+}
+
+hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipStream_t s)
+{
+    const int ki = P->KI, mt = P->MT_tmpl;
+    const size_t lds = P->lds_beam;
+    if (P->mode == 0) {
+        if (ki == 8) return fnk::launch_beamform_m0_k8(mt, P->wcr, P->wct, grid, lds, s, A);
+        if (ki == 16) return fnk::launch_beamform_m0_k16(mt, P->wcr, P->wct, grid, lds, s, A);
+        return fnk::launch_beamform_m0_k32(mt, P->wcr, P->wct, grid, lds, s, A);
+    }
+    if (ki == 8) return fnk::launch_beamform_m1_k8(mt, P->wcr, P->wct, grid, lds, s, A);
+    if (ki == 16) return fnk::launch_beamform_m1_k16(mt, P->wcr, P->wct, grid, lds, s, A);
+    return fnk::launch_beamform_m1_k32(mt, P->wcr, P->wct, grid, lds, s, A);
+}
+
+int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w, float *d_w_rx, hipStream_t user)
+{
+    if (!P || !d_rf || !d_image) return fail(FN_ERR_ARGUMENT, "null argument");
+    if (dtype != FN_DTYPE_F32 && dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
+    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
+    hipStream_t s = user ? user : P->stream;
+    fnk::BeamArgs A = P->args;
+    A.image = reinterpret_cast<float2 *>(d_image);
+    if (P->mode == 0) {
+        A.weights = d_w;
+        A.weights_tx = nullptr;
+    } else {
+        A.weights = d_w_rx;
+        A.weights_tx = d_w;
+    }
+    Run run{};
+    if (P->profiling) {
+        if (!P->spare.empty()) {
+            run = P->spare.back();
+            P->spare.pop_back();
+        } else {
+            HIP_CHECK(hipEventCreate(&run.e0), "hipEventCreate");
+            HIP_CHECK(hipEventCreate(&run.e1), "hipEventCreate");
+            HIP_CHECK(hipEventCreate(&run.e2), "hipEventCreate");
+        }
+    }
+    if (A.atomic_out) {
+        const long n = 2 * P->n_vox;
+        hipLaunchKernelGGL(fnk::fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_image, n);
+        HIP_CHECK(hipGetLastError(), "fill_zero_kernel");
+    }
+    if (P->profiling) HIP_CHECK(hipEventRecord(run.e0, s), "hipEventRecord");
+    const fnk::SpecArgs &S = P->sargs;
+    dim3 sg((unsigned)((S.n_f + 63) / 64), (unsigned)((S.n_ch + 63) / 64));
+    if (dtype == FN_DTYPE_F32) {
+        hipLaunchKernelGGL(fnk::spectra_kernel<float>, sg, dim3(256), P->lds_spec, s,
+                           reinterpret_cast<const float *>(d_rf), P->d_tw, P->d_X, S);
+    } else {
+        hipLaunchKernelGGL(fnk::spectra_kernel<double>, sg, dim3(256), P->lds_spec, s,
+                           reinterpret_cast<const double *>(d_rf), P->d_tw, P->d_X, S);
+    }
+    HIP_CHECK(hipGetLastError(), "spectra_kernel");
+    if (P->profiling) HIP_CHECK(hipEventRecord(run.e1, s), "hipEventRecord");
+    const long tiles = (P->n_vox + 63) / 64;
+    hipError_t e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups), A, s);
+    if (e != hipSuccess) return hip_fail(e, "beamform_kernel");
+    if (P->profiling) {
+        HIP_CHECK(hipEventRecord(run.e2, s), "hipEventRecord");
+        P->pending.push_back(run);
+    }
+    return FN_OK;
+}
+
+int collect_profile(fn_plan *P)
+{
+    for (auto &r : P->pending) {
+        HIP_CHECK(hipEventSynchronize(r.e2), "hipEventSynchronize");
+        float a = 0.f, b = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&a, r.e0, r.e1), "hipEventElapsedTime");
+        HIP_CHECK(hipEventElapsedTime(&b, r.e1, r.e2), "hipEventElapsedTime");
+        P->spec_ms += a;
+        P->beam_ms += b;
+        P->runs += 1;
+        P->spare.push_back(r);
+    }
+    P->pending.clear();
+    return FN_OK;
+}
+
+int pw_plan(const fn_pw_params *p, fn_plan **out)
+{
+    fnh::Grid g;
+    int s = fnh::validate_pw(p, &g);
+    if (s) return s;
+    if (p->normalization == FN_NORMALIZATION_WINDOWED)
+        return fail(FN_ERR_UNSUPPORTED, "normalizations.windowed is not available in this build");
+    fn_plan *P = new (std::nothrow) fn_plan();
+    if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
+    P->mode = 0;
+    int wsup = 0;
+    if ((s = setup_window(p->window, p->semantics, &P->args.wrx, &P->wcr, &wsup))) { destroy(P); return s; }
+    P->args.support_rx = wsup;
+    P->args.norm_rx = p->normalization != FN_NORMALIZATION_OFF;
+    P->args.clamp = p->semantics == FN_SEMANTICS_CPU;
+    Common c{p->positions_x, p->positions_z, p->n_x, p->n_z, p->n_samples, p->n_elements, p->n_angles,
+             p->f_s, p->element_width, p->element_pitch, p->c_0, p->index_t0,
+             p->semantics, p->precision, p->output_scale, p->index_gpu, p->verbosity};
+    if ((s = create_common(P, c, g, p->f_number_values, nullptr))) { destroy(P); return s; }
+    // steering geometry in FP64: das_pw.m:323-327 (x_ref = sign(e_x) x_0)
+    std::vector<double> txp(3 * (size_t)p->n_angles);
+    const double M = (p->n_elements - 1) / 2.0;
+    for (int a = 0; a < p->n_angles; ++a) {
+        const double th = p->steering_angles[a];
+        const double ex = -std::sin(th), ez = std::cos(th);
+        const double sg = (ex > 0) - (ex < 0);
+        txp[3 * a] = ex;
+        txp[3 * a + 1] = ez;
+        txp[3 * a + 2] = sg * ((0 - M) * p->element_pitch);
+    }
+    if ((s = upload(&P->d_txp, txp.data(), txp.size(), "steering"))) { destroy(P); return s; }
+    P->args.txp = P->d_txp;
+    P->useful_flops = P->flops * useful_fraction(g, p->positions_x, p->n_x, p->positions_z, p->n_z, p->f_number_values,
+                                                 p->f_s, p->c_0, p->element_pitch, p->n_elements, P->args.clamp != 0);
+    *out = P;
+    return FN_OK;
+}
+
+int sa_plan(const fn_sa_params *p, fn_plan **out)
+{
+    fnh::Grid g;
+    int s = fnh::validate_sa(p, &g);
+    if (s) return s;
+    if (p->normalization_tx == FN_NORMALIZATION_WINDOWED || p->normalization_rx == FN_NORMALIZATION_WINDOWED)
+        return fail(FN_ERR_UNSUPPORTED, "normalizations.windowed is not available in this build");
+    fn_plan *P = new (std::nothrow) fn_plan();
+    if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
+    P->mode = 1;
+    int sup = 0;
+    if ((s = setup_window(p->window_rx, p->semantics, &P->args.wrx, &P->wcr, &sup))) { destroy(P); return s; }
+    P->args.support_rx = sup;
+    if ((s = setup_window(p->window_tx, p->semantics, &P->args.wtx, &P->wct, &sup))) { destroy(P); return s; }
+    P->args.support_tx = sup;
+    P->args.norm_rx = p->normalization_rx != FN_NORMALIZATION_OFF;
+    P->args.norm_tx = p->normalization_tx != FN_NORMALIZATION_OFF;
+    P->args.clamp = 0;  // das_sa.m:354-358: no depth clamp
+    Common c{p->positions_x, p->positions_z, p->n_x, p->n_z, p->n_samples, p->n_elements, p->n_elements,
+             p->f_s, p->element_width, p->element_pitch, p->c_0, p->index_t0,
+             p->semantics, p->precision, p->output_scale, p->index_gpu, p->verbosity};
+    if ((s = create_common(P, c, g, p->f_number_rx_values, p->f_number_tx_values))) { destroy(P); return s; }
+    P->useful_flops = P->flops * useful_fraction(g, p->positions_x, p->n_x, p->positions_z, p->n_z,
+                                                 p->f_number_rx_values, p->f_s, p->c_0, p->element_pitch,
+                                                 p->n_elements, false);
+    *out = P;
+    return FN_OK;
+}
+
+size_t dtype_size(int dtype) { return dtype == FN_DTYPE_F64 ? 8 : 4; }
+
+// one-shot: H2D, execute, D2H (gpu_bf_das_pw_rf.cu:730-1122)
+int one_shot(fn_plan *P, const void *rf, int dtype, float *image, float *w0, float *w1)
+{
+    const size_t rf_bytes = (size_t)P->n_t * P->n_el * P->n_tx * dtype_size(dtype);
+    const size_t img_bytes = (size_t)P->n_vox * 2 * sizeof(float);
+    const size_t w_bytes = (size_t)P->n_vox * sizeof(float);
+    void *d_rf = nullptr;
+    float *d_img = nullptr, *d_w0 = nullptr, *d_w1 = nullptr;
+    int s = FN_OK;
+    hipError_t e = hipMalloc(&d_rf, rf_bytes);
+    if (e == hipSuccess) e = hipMalloc(&d_img, img_bytes);
+    if (e == hipSuccess && w0) e = hipMalloc(&d_w0, w_bytes);
+    if (e == hipSuccess && w1) e = hipMalloc(&d_w1, w_bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_rf, rf, rf_bytes, hipMemcpyHostToDevice, P->stream);
+    if (e != hipSuccess) s = hip_fail(e, "one-shot buffers");
+    if (!s) s = execute(P, d_rf, dtype, d_img, d_w0, d_w1, nullptr);
+    if (!s) {
+        e = hipMemcpyAsync(image, d_img, img_bytes, hipMemcpyDeviceToHost, P->stream);
+        if (e == hipSuccess && w0) e = hipMemcpyAsync(w0, d_w0, w_bytes, hipMemcpyDeviceToHost, P->stream);
+        if (e == hipSuccess && w1) e = hipMemcpyAsync(w1, d_w1, w_bytes, hipMemcpyDeviceToHost, P->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+        if (e != hipSuccess) s = hip_fail(e, "one-shot results");
+    }
+    hipFree(d_rf); hipFree(d_img); hipFree(d_w0); hipFree(d_w1);
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t fn_device_count(int32_t *count)
+{
+    if (!count) return fail(FN_ERR_ARGUMENT, "null argument");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    *count = (e == hipSuccess) ? c : 0;
+    return FN_OK;
+}
+
+int32_t fn_plan_pw_create(const fn_pw_params *p, fn_plan **plan)
+{
+    if (!plan) return fail(FN_ERR_ARGUMENT, "null plan pointer");
+    *plan = nullptr;
+    return pw_plan(p, plan);
+}
+
+int32_t fn_plan_sa_create(const fn_sa_params *p, fn_plan **plan)
+{
+    if (!plan) return fail(FN_ERR_ARGUMENT, "null plan pointer");
+    *plan = nullptr;
+    return sa_plan(p, plan);
+}
+
+int32_t fn_plan_info_get(const fn_plan *P, fn_plan_info *info)
+{
+    if (!P || !info) return fail(FN_ERR_ARGUMENT, "null argument");
+    info->N_dft = P->g.N;
+    info->n_lb = P->g.n_lb;
+    info->n_ub = P->g.n_ub;
+    info->n_f = P->g.n_f;
+    info->n_tx = P->n_tx;
+    info->n_voxels = P->n_vox;
+    info->workspace_bytes = (int64_t)P->x_bytes;
+    info->flops_per_frame = P->flops;
+    info->useful_flops_per_frame = P->useful_flops;
+    info->tile_voxels = 64;
+    info->bin_split = P->bin_split;
+    info->m_tiles = P->MT_tmpl;
+    info->k_steps = P->KI;
+    return FN_OK;
+}
+
+int32_t fn_plan_execute(fn_plan *P, const void *d_rf, int32_t dtype, float *d_image, float *d_w, float *d_w_rx,
+                        void *stream)
+{
+    fnh::clear_error();
+    return execute(P, d_rf, dtype, d_image, d_w, d_w_rx, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t fn_plan_set_profiling(fn_plan *P, int32_t enable)
+{
+    if (!P) return fail(FN_ERR_ARGUMENT, "null plan");
+    P->profiling = enable ? 1 : 0;
+    return FN_OK;
+}
+
+int32_t fn_plan_get_profile(fn_plan *P, double *spectra_ms, double *beamform_ms, int32_t *n_runs)
+{
+    if (!P) return fail(FN_ERR_ARGUMENT, "null plan");
+    int s = collect_profile(P);
+    if (s) return s;
+    if (spectra_ms) *spectra_ms = P->spec_ms;
+    if (beamform_ms) *beamform_ms = P->beam_ms;
+    if (n_runs) *n_runs = P->runs;
+    P->spec_ms = P->beam_ms = 0.0;
+    P->runs = 0;
+    return FN_OK;
+}
+
+int32_t fn_plan_destroy(fn_plan *P)
+{
+    if (P) {
+        hipSetDevice(P->device);
+        hipStreamSynchronize(P->stream);
+    }
+    destroy(P);
+    return FN_OK;
+}
+
+int32_t fn_das_pw(const fn_pw_params *p, const void *data_rf, int32_t rf_dtype, float *image, float *weights)
+{
+    fnh::clear_error();
+    if (!data_rf) return fail(FN_ERR_RF_DATA, "RF data missing");
+    if (!image) return fail(FN_ERR_ARGUMENT, "image buffer missing");
+    if (rf_dtype != FN_DTYPE_F32 && rf_dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
+    fn_plan *P = nullptr;
+    int s = pw_plan(p, &P);
+    if (s) return s;
+    s = one_shot(P, data_rf, rf_dtype, image, weights, nullptr);
+    destroy(P);
+    return s;
+}
+
+int32_t fn_das_sa(const fn_sa_params *p, const void *data_rf, int32_t rf_dtype, float *image, float *weights_tx,
+                  float *weights_rx)
+{
+    fnh::clear_error();
+    if (!data_rf) return fail(FN_ERR_RF_DATA, "RF data missing");
+    if (!image) return fail(FN_ERR_ARGUMENT, "image buffer missing");
+    if (rf_dtype != FN_DTYPE_F32 && rf_dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
+    fn_plan *P = nullptr;
+    int s = sa_plan(p, &P);
+    if (s) return s;
+    s = one_shot(P, data_rf, rf_dtype, image, weights_tx, weights_rx);
+    destroy(P);
+    return s;
+}
+
+}  // extern "C"

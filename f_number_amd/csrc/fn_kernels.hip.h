@@ -1,0 +1,412 @@
+// fn_kernels.hip.h -- device code of libfnumber_hip.so for gfx950 (CDNA4).
+//
+// Two kernels form the hot path of das_pw / das_sa:
+//
+//  spectra_kernel   band-limited real-to-complex DFT of every RF channel
+//                   (replaces cuFFT in gpu_bf_das_pw_rf.cu:917-925 and the
+//                   bin selection of das_pw.m:383-384).  Twiddle index
+//                   (n t) mod N is reduced exactly in integers; the output is
+//                   written directly in the MFMA operand order the
+//                   beamforming kernel stages into LDS.
+//
+//  beamform_kernel  for every band bin n: generates the receive
+//                   apodization x phase tile B_n(v,k) = a_{n,k}(v) e^{j w_n tau_k(v)}
+//                   in registers (replaces das_kernel_phase_shifts.cu:8-147 /
+//                   saft_kernel_phase_shifts.cu:8-199, which wrote it to HBM),
+//                   contracts it against the transmit spectra X_n(k,a) on
+//                   v_mfma_f32_16x16x4f32 (replaces the cublasCgemm GEMV of
+//                   gpu_bf_das_pw_rf.cu:1034-1041), then applies the transmit
+//                   phase / transmit apodization, the normalization
+//                   (das_kernel_normalization.cu:8-34) and accumulates the
+//                   image over bins in registers.
+//
+// Phases: tau is formed in FP64 and reduced to q = frac(f_s tau / N) in
+// units of 2^-32 (uint32).  The phase of bin n is then the exact wrap-around
+// integer product n*Q mod 2^32, converted to revolutions and evaluated with
+// the hardware sine/cosine (v_sin_f32/v_cos_f32 take revolutions).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fnk {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// window classes for the element loop (the generic window below covers the
+// rest with runtime coefficients)
+enum { WC_RANGE = 0, WC_LINEAR = 1, WC_COS = 2 };
+enum { MODE_PW = 0, MODE_SA = 1 };
+
+// Generic |t| window: 1 for t <= t_flat; for t_flat < t < t_cut:
+// A + B cos(pi (t - t_off) s) (cosine class) or A + C t (linear class);
+// 0 for t >= t_cut.  Coefficients per window (CPU semantics from
+// +windows/<name>.m, MEX semantics from gpu_bf_windows.cu):
+//   tukey(alpha): t_flat = t_off = 1 - alpha, t_cut = 1, A = B = 1/2, s = 1/alpha
+//   hann:         t_flat = -1, t_off = 0, t_cut = inf (CPU) | 1 (MEX), A = B = 1/2, s = 1
+//   triangular:   t_flat = -1, t_cut = inf | 1, A = 1, B = 0, C = -1 (linear class)
+//   boxcar:       constant 1 on its integer support (range class)
+struct WinCoef {
+    float t_flat, t_cut, t_off, A, B, C, s_half;  // cosine argument: (t - t_off) * s_half revolutions
+};
+
+// element support of a window for one aperture
+enum { SUPPORT_OPEN_H = 0,   // boxcar, CPU: |x_k - x| / h < 1 (das_pw.m:441)
+       SUPPORT_SNAPPED = 1,  // [k_lb, k_ub] (MEX: apply() truncation, das_kernel_phase_shifts.cu:94-125)
+       SUPPORT_ALL = 2 };    // CPU non-boxcar windows: every element (das_pw.m:449-452)
+
+struct BeamArgs {
+    const float2 *X;        // spectra in operand order [n_f][MT][KI][64]
+    const double *px, *pz;  // voxel positions
+    const double *Frx;      // F-number per band bin (receive)
+    const double *Ftx;      // F-number per band bin (transmit, SA)
+    const double *txp;      // PW: per angle (e_x, e_z, x_ref)
+    float2 *image;
+    float *weights, *weights_tx;
+    int n_x, n_z, n_el, n_tx, n_f, n_lb, N;
+    int bin_chunk;          // bins per blockIdx.y
+    int tx_group;           // M-tiles per blockIdx.z group (PW angle groups)
+    int clamp;              // CPU-semantics depth clamp of the PW F-number (das_pw.m:374-377)
+    int support_rx, support_tx;        // SUPPORT_* of the receive / transmit window
+    int norm_rx, norm_tx;
+    int atomic_out;
+    double M, pitch, width, c0, f_s, t0;
+    WinCoef wrx, wtx;
+    float out_scale;
+};
+
+struct SpecArgs {
+    int n_t, n_ch, N, n_lb, n_f, n_el, n_tx, MT, KI;
+};
+
+__device__ __forceinline__ uint32_t frac32(double q)
+{
+    const double f = q - floor(q);
+    return (uint32_t)(int64_t)rint(f * 4294967296.0);
+}
+
+__device__ __forceinline__ float rev_of(uint32_t P)
+{
+    return (float)(int32_t)P * 2.3283064365386963e-10f;  // 2^-32
+}
+
+// Per (voxel, bin) aperture: das_pw.m:387, 399-411 (CPU) and
+// das_kernel_phase_shifts.cu:90-125 (MEX).  All bounds in FP64, as the oracle.
+struct Aperture {
+    int klo, khi;           // element support (inclusive); empty when klo > khi
+    float inv_wl, inv_wr;   // |1 / W_L|, |1 / W_R| (snapped half-widths)
+    bool valid;             // non-empty aperture: the bin contributes
+};
+
+__device__ __forceinline__ Aperture make_aperture(double x, double z, double F, const BeamArgs &A, int support)
+{
+    Aperture ap;
+    const double h = z / (2.0 * F);
+    const double ua = A.M + (x - h) / A.pitch;
+    const double ub = A.M + (x + h) / A.pitch;
+    const double lbd = fmax(ceil(ua), 0.0);
+    const double ubd = fmin(floor(ub), (double)(A.n_el - 1));
+    ap.valid = lbd <= ubd;
+    ap.klo = 1;
+    ap.khi = 0;
+    ap.inv_wl = ap.inv_wr = 0.0f;
+    if (ap.valid) {
+        const int lb = (int)lbd, ubi = (int)ubd;
+        const double wl = x - (((double)lb - A.M) * A.pitch - A.width / 2);
+        const double wr = (((double)ubi - A.M) * A.pitch + A.width / 2) - x;
+        ap.inv_wl = (float)fabs(1.0 / wl);
+        ap.inv_wr = (float)fabs(1.0 / wr);
+        if (support == SUPPORT_OPEN_H) {
+            // {k : |x_k - x| / h < 1} = open interval (ua, ub) in element units
+            const double lo = fmax(floor(ua) + 1.0, 0.0);
+            const double hi = fmin(ceil(ub) - 1.0, (double)(A.n_el - 1));
+            if (lo <= hi) { ap.klo = (int)lo; ap.khi = (int)hi; }
+        } else if (support == SUPPORT_SNAPPED) {
+            ap.klo = lb;
+            ap.khi = ubi;
+        } else {
+            ap.klo = 0;
+            ap.khi = A.n_el - 1;
+        }
+    }
+    return ap;
+}
+
+template <int WC>
+__device__ __forceinline__ float window_eval(float t, const WinCoef &w)
+{
+    if (WC == WC_RANGE) return 1.0f;
+    float v;
+    if (WC == WC_LINEAR) {
+        v = w.A + w.C * t;
+    } else {
+        float u = (t - w.t_off) * w.s_half;   // revolutions
+        u = u - rintf(u);                     // keep v_cos_f32 in its reduced range
+        v = w.A + w.B * __builtin_amdgcn_cosf(u);
+    }
+    v = (t <= w.t_flat) ? 1.0f : v;
+    v = (t < w.t_cut) ? v : 0.0f;
+    return v;
+}
+
+// -------------------------------------------------------------------------
+// band-limited DFT
+// -------------------------------------------------------------------------
+// block: 256 threads, tile 64 bins x 64 channels, sample chunks of 64.
+// thread: 4 bins (stride 16) x 4 channels (stride 16).
+template <typename T>
+__global__ __launch_bounds__(256) void spectra_kernel(const T *__restrict__ rf, const float2 *__restrict__ twiddle,
+                                                      float2 *__restrict__ X, SpecArgs S)
+{
+    extern __shared__ float smem[];
+    float2 *tw = reinterpret_cast<float2 *>(smem);       // [N]
+    float *xs = smem + 2 * S.N;                          // [64][65]
+    const int tid = threadIdx.x;
+    const int bi = tid & 15, cg = tid >> 4;
+    const int b0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    for (int m = tid; m < S.N; m += 256) tw[m] = twiddle[m];
+
+    int nq[4], mq[4];
+    float ar[4][4], ai[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int b = b0 + bi + 16 * q;
+        nq[q] = (b < S.n_f) ? S.n_lb + b : 0;  // bins past the band are computed as n = 0 and dropped
+        mq[q] = 0;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) { ar[q][p] = 0.f; ai[q][p] = 0.f; }
+    }
+    for (int t0 = 0; t0 < S.n_t; t0 += 64) {
+        __syncthreads();
+        for (int e = tid; e < 64 * 64; e += 256) {
+            const int cl = e >> 6, tl = e & 63;
+            const int ch = c0 + cl, t = t0 + tl;
+            float v = 0.f;
+            if (ch < S.n_ch && t < S.n_t) v = (float)rf[(size_t)ch * S.n_t + t];
+            xs[cl * 65 + tl] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) mq[q] = (int)(((int64_t)nq[q] * t0) % S.N);
+        __syncthreads();
+        const int tn = min(64, S.n_t - t0);
+        for (int tl = 0; tl < tn; ++tl) {
+            float xv[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) xv[p] = xs[(cg + 16 * p) * 65 + tl];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float2 w = tw[mq[q]];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    ar[q][p] = fmaf(xv[p], w.x, ar[q][p]);
+                    ai[q][p] = fmaf(-xv[p], w.y, ai[q][p]);
+                }
+                int m = mq[q] + nq[q];
+                mq[q] = (m >= S.N) ? m - S.N : m;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int b = b0 + bi + 16 * q;
+        if (b >= S.n_f) continue;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int ch = c0 + cg + 16 * p;
+            if (ch >= S.n_ch) continue;
+            const int a = ch / S.n_el, k = ch - a * S.n_el;
+            const size_t off = (((size_t)b * S.MT + (a >> 4)) * S.KI + (k >> 2)) * 64 + (a & 15) + 16 * (k & 3);
+            X[off] = make_float2(ar[q][p], ai[q][p]);
+        }
+    }
+}
+
+// -------------------------------------------------------------------------
+// fused beamforming
+// -------------------------------------------------------------------------
+// block: 256 threads = 4 waves; wave = 16 voxels (MFMA N), lane group g = lane/16
+// handles elements k = 4j + g.  M rows of the MFMA are transmit indices
+// (angles for PW, transmitting elements for SA), 16 per M-tile.
+// Lane l holds D[4g + r][l & 15] = Y(tx = 16 mt + 4g + r, voxel l & 15).
+template <int MODE, int MT, int KI, int WCR, int WCT>
+__global__ __launch_bounds__(256) void beamform_kernel(BeamArgs A)
+{
+    extern __shared__ float2 xs[];  // [MT][KI][64]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, c = lane & 15;
+    const long n_vox = (long)A.n_x * A.n_z;
+    const long v = (long)blockIdx.x * 64 + wave * 16 + c;
+    const bool vok = v < n_vox;
+    const long vv = vok ? v : n_vox - 1;
+    const int ix = (int)(vv / A.n_z), iz = (int)(vv - (long)ix * A.n_z);
+    const double x = A.px[ix], z = A.pz[iz];
+    const int mt0 = blockIdx.z * A.tx_group;  // first M-tile of this transmit group
+
+    // receive phase increments: tau_rx = sqrt(((x_k-x)/c)^2 + (z/c)^2) (das_pw.m:339-340,427)
+    const double zc = z / A.c0;
+    const double t_half = (MODE == MODE_SA) ? 0.5 * A.t0 / A.f_s : 0.0;  // das_sa.m:431
+    uint32_t Qrx[KI];
+#pragma unroll
+    for (int j = 0; j < KI; ++j) {
+        const int k = 4 * j + g;
+        const double lat = (((double)k - A.M) * A.pitch - x) / A.c0;
+        const double tau = sqrt(lat * lat + zc * zc) + t_half;
+        Qrx[j] = frac32(A.f_s * tau / A.N);
+    }
+    // transmit phase increments for the rows this lane holds
+    uint32_t Qtx[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int a = 16 * (mt0 + mt) + 4 * g + r;
+            double tau = 0.0;
+            if (MODE == MODE_PW) {
+                // das_pw.m:336: (e_x (x - x_ref) + e_z z) / c + i_t0 / f_s
+                if (a < A.n_tx) {
+                    const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
+                    tau = (ex * (x - xr) + ez * z) / A.c0 + A.t0 / A.f_s;
+                }
+            } else {
+                if (a < A.n_tx) {
+                    const double lat = (((double)a - A.M) * A.pitch - x) / A.c0;
+                    tau = sqrt(lat * lat + zc * zc) + t_half;
+                }
+            }
+            Qtx[mt][r] = frac32(A.f_s * tau / A.N);
+        }
+    const float dg = (float)(((double)g - A.M) * A.pitch - x);   // x_k - x for j = 0
+    const float dstep = (float)(4.0 * A.pitch);
+    const float dg_tx = (float)((4.0 * g - A.M) * A.pitch - x);  // SA transmit rows, mt = r = 0
+
+    float Ir = 0.f, Ii = 0.f, w_last = 0.f, wtx_last = 0.f;
+    const int b_begin = blockIdx.y * A.bin_chunk;
+    const int b_end = min(b_begin + A.bin_chunk, A.n_f);
+    const int n_f4 = MT * KI * 32;  // float4 per bin
+    const float4 *Xg = reinterpret_cast<const float4 *>(A.X);
+    const int MT_total = (A.n_tx + 15) >> 4;
+    const int MT_all = MT_total;  // stride of the operand layout
+
+    for (int b = b_begin; b < b_end; ++b) {
+        const int n = A.n_lb + b;
+        __syncthreads();
+        {
+            // stage this group's M-tiles of X_n into LDS
+            const float4 *src = Xg + ((size_t)b * MT_all + mt0) * KI * 32;
+            float4 *dst = reinterpret_cast<float4 *>(xs);
+            const int n4 = min(MT, MT_all - mt0) * KI * 32;
+            for (int e = threadIdx.x; e < n_f4; e += 256) dst[e] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+
+        const double f_n = A.f_s * (double)n / A.N;  // das_pw.m:361
+        double Fr = A.Frx[b];
+        if (A.clamp) Fr = fmin(Fr, sqrt(f_n * z / (2.88 * A.c0)));  // das_pw.m:374-377
+        const Aperture ap = make_aperture(x, z, Fr, A, A.support_rx);
+
+        floatx4 accR[MT], accI[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            accR[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+            accI[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        float asum = 0.f;
+        const int klo = ap.klo - g, khi = ap.khi - g;  // support in units of 4j
+#pragma unroll
+        for (int j = 0; j < KI; ++j) {
+            const float d = fmaf((float)j, dstep, dg);
+            const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
+            float a = window_eval<WCR>(t, A.wrx);
+            a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
+            const uint32_t P = (uint32_t)n * Qrx[j];
+            const float rv = rev_of(P);
+            const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+            const float br = a * cs, bi = a * sn, nbi = -bi;
+            asum += a;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const float2 xv = xs[(mt * KI + j) * 64 + lane];
+                accR[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, br, accR[mt], 0, 0, 0);
+                accI[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, bi, accI[mt], 0, 0, 0);
+                accR[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, nbi, accR[mt], 0, 0, 0);
+                accI[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, br, accI[mt], 0, 0, 0);
+            }
+        }
+        asum += __shfl_xor(asum, 16);
+        asum += __shfl_xor(asum, 32);
+
+        if (MODE == MODE_PW) {
+            if (ap.valid) {
+                const float sc = A.norm_rx ? 1.0f / asum : 1.0f;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int a = 16 * (mt0 + mt) + 4 * g + r;
+                        if (a < A.n_tx) {
+                            const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
+                            const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                            const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
+                            Ir += cs * yr - sn * yi;
+                            Ii += cs * yi + sn * yr;
+                        }
+                    }
+            }
+        } else {
+            // transmit aperture and apodization (das_sa.m:384-400, 438-455)
+            const double Ft = A.Ftx[b];
+            const Aperture at = make_aperture(x, z, Ft, A, A.support_tx);
+            float wtx[MT][4];
+            float tsum = 0.f;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * (mt0 + mt) + 4 * g + r;
+                    const float d = fmaf((float)(16 * (mt0 + mt) + r), (float)A.pitch, dg_tx);
+                    const float t = fabsf(d) * (d < 0.f ? at.inv_wl : at.inv_wr);
+                    float w = window_eval<WCT>(t, A.wtx);
+                    w = (i >= at.klo && i <= at.khi) ? w : 0.f;
+                    wtx[mt][r] = w;
+                    tsum += w;
+                }
+            tsum += __shfl_xor(tsum, 16);
+            tsum += __shfl_xor(tsum, 32);
+            if (ap.valid && at.valid) {
+                const float sc = (A.norm_rx ? 1.0f / asum : 1.0f) * (A.norm_tx ? 1.0f / tsum : 1.0f);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
+                        const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                        const float w = wtx[mt][r] * sc;
+                        const float yr = accR[mt][r] * w, yi = accI[mt][r] * w;
+                        Ir += cs * yr - sn * yi;
+                        Ii += cs * yi + sn * yr;
+                    }
+            }
+            if (b == A.n_f - 1) wtx_last = tsum;
+        }
+        if (b == A.n_f - 1) w_last = asum;
+    }
+    Ir += __shfl_xor(Ir, 16);
+    Ir += __shfl_xor(Ir, 32);
+    Ii += __shfl_xor(Ii, 16);
+    Ii += __shfl_xor(Ii, 32);
+    if (g == 0 && vok) {
+        const float2 val = make_float2(A.out_scale * Ir, A.out_scale * Ii);
+        if (A.atomic_out) {
+            atomicAdd(&A.image[v].x, val.x);
+            atomicAdd(&A.image[v].y, val.y);
+        } else {
+            A.image[v] = val;
+        }
+        if (b_end == A.n_f && blockIdx.z == 0) {
+            if (A.weights) A.weights[v] = w_last;
+            if (MODE == MODE_SA && A.weights_tx) A.weights_tx[v] = wtx_last;
+        }
+    }
+}
+
+}  // namespace fnk

@@ -1,0 +1,251 @@
+/*
+ * fnumber_hip.h -- C ABI of libfnumber_hip.so, the MI355X (gfx950) Fourier-domain
+ * delay-and-sum beamformer with frequency-dependent F-number.
+ *
+ * This is the drop-in boundary under the reference's MEX gateways:
+ *   +cuda/gpu_bf_das_pw_rf.cu:110  mexFunction (steered plane waves)
+ *   +cuda/gpu_bf_saft_rf.cu:107    mexFunction (full synthetic aperture)
+ * A MEX shim (f_number_amd/csrc/mex/) parses the reference's positional
+ * argument lists, evaluates MATLAB-side objects (windows, F-numbers,
+ * normalizations) into the plain values below and calls fn_das_pw/fn_das_sa.
+ * Every entry point returns an fn_status (0 = success) and never throws across
+ * the ABI; fn_status_id() gives the reference's MEX error identifier suffix
+ * ("InvalidRFData", ...) and fn_last_error() a message (thread-local).
+ *
+ * Conventions
+ *   - Images are complex float, interleaved (re, im), laid out [n_x][n_z]
+ *     (z fastest), i.e. MATLAB's column-major [N_z x N_x]
+ *     (gpu_bf_das_pw_rf.cu:1077-1079).  Weights are float [n_x][n_z].
+ *   - RF data are column-major [n_samples x n_elements x n_tx] (n_tx = steering
+ *     angles for PW, transmitting elements for SA), float32 or float64.
+ *   - Frequency bins are 0-based DFT indices n of order N_dft.
+ *   - No function allocates memory the caller must free, except plans
+ *     (fn_plan_destroy).
+ */
+#ifndef FNUMBER_HIP_H
+#define FNUMBER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FN_ABI_VERSION 1
+
+/* ---- status codes (MEX error identifiers: gpu_bf_das_pw_rf.cu:112-485,
+ *      gpu_bf_saft_rf.cu:111-592, gpu_bf_error_handling.cu:9-70) ---------- */
+typedef enum fn_status {
+    FN_OK = 0,
+    FN_ERR_NRHS = 1,                     /* "nrhs" */
+    FN_ERR_NLHS = 2,                     /* "nlhs" */
+    FN_ERR_NO_DEVICES = 3,               /* "NoCUDADevices" */
+    FN_ERR_LATERAL_POSITIONS = 4,        /* "InvalidLateralPositions" */
+    FN_ERR_AXIAL_POSITIONS = 5,          /* "InvalidAxialPositions" */
+    FN_ERR_RF_DATA = 6,                  /* "InvalidRFData" */
+    FN_ERR_SAMPLING_RATE = 7,            /* "InvalidSamplingRate" */
+    FN_ERR_STEERING_ANGLE = 8,           /* "InvalidSteeringAngle" */
+    FN_ERR_ELEMENT_WIDTH = 9,            /* "InvalidElementWidth" */
+    FN_ERR_ELEMENT_PITCH = 10,           /* "InvalidElementPitch" */
+    FN_ERR_SOUND_SPEED = 11,             /* "InvalidSoundSpeed" */
+    FN_ERR_FREQUENCY_BOUNDS = 12,        /* "InvalidFrequencyBounds" */
+    FN_ERR_FREQUENCY_BOUND = 13,         /* "InvalidFrequencyBound" */
+    FN_ERR_TIME_SHIFT = 14,              /* "InvalidTimeShift" */
+    FN_ERR_WINDOW = 15,                  /* "InvalidWindow" */
+    FN_ERR_F_NUMBER = 16,                /* "InvalidFNumber" */
+    FN_ERR_NORMALIZATION = 17,           /* "InvalidNormalization" */
+    FN_ERR_FREQUENCY_WINDOW = 18,        /* "InvalidFrequencyWindow" */
+    FN_ERR_DEVICE = 19,                  /* "InvalidCUDADevice" (id kept for drop-in) */
+    FN_ERR_VERBOSITY = 20,               /* "InvalidVerbosityLevel" */
+    FN_ERR_TX_F_NUMBER = 21,             /* "InvalidTXFNumber" */
+    FN_ERR_RX_F_NUMBER = 22,             /* "InvalidRXFNumber" */
+    FN_ERR_HIP = 23,                     /* runtime failure: "CUDAError" (id kept) */
+    FN_ERR_OUT_OF_MEMORY = 24,           /* "OutOfMemory" */
+    FN_ERR_ARGUMENT = 25,                /* "InvalidArgument": null pointer, bad enum, size */
+    FN_ERR_UNSUPPORTED = 26              /* "Unsupported": a valid request this build cannot serve */
+} fn_status;
+
+/* window ids: the MEX numbering (gpu_bf_windows.cuh:14-20; the class map of
+ * gpu_bf_helper_functions.cu:56-97 sends windows.triangular to 4) */
+enum { FN_WINDOW_BOXCAR = 0, FN_WINDOW_HANN = 1, FN_WINDOW_TUKEY = 2, FN_WINDOW_COSINE = 3,
+       FN_WINDOW_TRIANGULAR = 4 };
+/* normalizations (+normalizations/on.m, off.m, windowed.m) */
+enum { FN_NORMALIZATION_OFF = 0, FN_NORMALIZATION_ON = 1, FN_NORMALIZATION_WINDOWED = 2 };
+/* semantics: CPU follows das_pw.m/das_sa.m (N_pad from positions_z(1), depth clamp
+ * of the PW F-number, untruncated Hann/triangular, continuous boxcar);
+ * MEX follows the +cuda kernels (positions_z[1], no clamp, every window
+ * truncated at |t| >= 1, element-snapped boxcar). */
+enum { FN_SEMANTICS_CPU = 0, FN_SEMANTICS_MEX = 1 };
+/* arithmetic of the element x transmit contraction */
+enum { FN_PRECISION_FP32 = 0,    /* FP32 MFMA operands, FP32 accumulation */
+       FN_PRECISION_BF16X3 = 1,  /* BF16 hi+lo split operands, 3 MFMA passes, FP32 accumulation */
+       FN_PRECISION_FP16X3 = 2,  /* FP16 hi+lo split operands, 3 passes */
+       FN_PRECISION_FP16 = 3 };  /* FP16 operands, 1 pass */
+/* output scale: MEX returns 2/N_dft * sum (gpu_bf_das_pw_rf.cu:1118-1119);
+ * the MATLAB CPU path returns 2 * sum (das_pw.m:384,467) */
+enum { FN_SCALE_MEX = 0, FN_SCALE_CPU = 1 };
+enum { FN_DTYPE_F32 = 0, FN_DTYPE_F64 = 1 };
+
+typedef struct fn_window {
+    int32_t id;          /* FN_WINDOW_* */
+    double parameter;    /* Tukey cosine fraction in (0,1); ignored otherwise */
+} fn_window;
+
+/* F-number objects of the +f_numbers package (C++ port, replaces the
+ * mexCallMATLAB round trip of gpu_bf_helper_functions.cu:35-51). */
+enum {
+    FN_F_NUMBER_CONSTANT = 0,            /* constant(value): p0 = value */
+    FN_F_NUMBER_GRATING_ANGLE_LB = 1,    /* grating.angle_lb(chi_deg, F_ub, delta_deg): p0,p1,p2 */
+    FN_F_NUMBER_DISTANCE_CONSTANT = 2,   /* distance.constant(delta_deg, F_ub): p0,p1 (F_ub may be inf) */
+    FN_F_NUMBER_SZABO = 3,               /* directivity.szabo(width_over_pitch): p0 */
+    FN_F_NUMBER_DIRECTIVITY_SOFT = 4,    /* directivity.soft(width_over_pitch, att_dB): p0,p1 */
+    FN_F_NUMBER_DIRECTIVITY_RIGID = 5,   /* directivity.rigid(width_over_pitch, att_dB): p0,p1 */
+    FN_F_NUMBER_PERROT = 6,              /* directivity.perrot(width_over_pitch): p0 */
+    FN_F_NUMBER_DISTANCE_NONE = 7        /* distance.none: no parameters */
+};
+typedef struct fn_f_number {
+    int32_t kind;
+    double p0, p1, p2;
+} fn_f_number;
+
+/* Steered plane-wave beamforming (das_pw.m:1, gpu_bf_das_pw_rf.cu:110).
+ * steering_angles holds n_angles angles; the result is the compounded image
+ * sum_a das_pw(angle a) (examples.m:70-78).  One angle with 2-D RF is the
+ * reference call exactly. */
+typedef struct fn_pw_params {
+    uint32_t struct_size;            /* sizeof(fn_pw_params) */
+    const double *positions_x;       /* [n_x] lateral voxel positions (m) */
+    int32_t n_x;
+    const double *positions_z;       /* [n_z] axial voxel positions (m), > 0 */
+    int32_t n_z;
+    int32_t n_samples;               /* N_t */
+    int32_t n_elements;              /* N_el */
+    int32_t n_angles;                /* N_a >= 1 */
+    const double *steering_angles;   /* [n_angles] rad, in (-pi/2, pi/2) */
+    double f_s, element_width, element_pitch, c_0;
+    double f_lb, f_ub;               /* Hz; the MEX defaults are DBL_MIN and f_s/2 */
+    double index_t0;                 /* >= 0 */
+    fn_window window;
+    const double *f_number_values;   /* [n_f] F per band bin n_lb..n_ub (fn_pw_frequency_grid) */
+    int32_t n_f;
+    int32_t normalization;           /* FN_NORMALIZATION_* */
+    fn_window normalization_window;  /* window of normalizations.windowed */
+    int32_t semantics;               /* FN_SEMANTICS_* */
+    int32_t precision;               /* FN_PRECISION_* */
+    int32_t output_scale;            /* FN_SCALE_* */
+    int32_t index_gpu;
+    int32_t verbosity;               /* 0..3 (gpu_bf_das_pw_rf.cu:469-485) */
+} fn_pw_params;
+
+/* Full synthetic aperture (das_sa.m:1, gpu_bf_saft_rf.cu:107).
+ * RF is [n_samples x n_elements(rx) x n_elements(tx)]. */
+typedef struct fn_sa_params {
+    uint32_t struct_size;
+    const double *positions_x;
+    int32_t n_x;
+    const double *positions_z;
+    int32_t n_z;
+    int32_t n_samples;
+    int32_t n_elements;
+    double f_s, element_width, element_pitch, c_0;
+    double f_lb, f_ub;
+    double index_t0;
+    fn_window window_tx, window_rx;
+    const double *f_number_tx_values;   /* [n_f] */
+    const double *f_number_rx_values;   /* [n_f] */
+    int32_t n_f;
+    int32_t normalization_tx, normalization_rx;
+    fn_window normalization_window_tx, normalization_window_rx;
+    int32_t semantics, precision, output_scale, index_gpu, verbosity;
+} fn_sa_params;
+
+/* ---- library ---------------------------------------------------------- */
+int32_t fn_abi_version(void);
+const char *fn_status_id(int32_t status);
+const char *fn_last_error(void);
+/* number of HIP devices (gpu_bf_das_pw_rf.cu:115-121) */
+int32_t fn_device_count(int32_t *count);
+
+/* ---- host-side configuration (no GPU involved) ------------------------- */
+/* Frequency grid: N_pad, odd N_dft and the band [n_lb, n_ub]
+ * (das_pw.m:343-358, gpu_bf_das_pw_rf.cu:525-552). n_f = n_ub - n_lb + 1. */
+int32_t fn_pw_frequency_grid(const fn_pw_params *p, int32_t *N_dft, int32_t *n_lb, int32_t *n_ub);
+int32_t fn_sa_frequency_grid(const fn_sa_params *p, int32_t *N_dft, int32_t *n_lb, int32_t *n_ub);
+/* element pitch over wavelength p*f_n/c_0 for the band bins: the argument the
+ * reference passes to compute_values (das_pw.m:368, gpu_bf_das_pw_rf.cu:562-575). */
+int32_t fn_element_pitch_over_lambda(double element_pitch, double f_s, double c_0, int32_t N_dft,
+                                     int32_t n_lb, int32_t n_ub, double *out);
+/* compute_values of an F-number object (+f_numbers/f_number.m:50-114). */
+int32_t fn_f_number_values(const fn_f_number *F, const double *element_pitch_over_lambda, int32_t n,
+                           double *out);
+/* window samples of t (|t| is taken; +windows/window.m:48-93 for CPU
+ * semantics, gpu_bf_windows.cu:15-103 for MEX semantics). */
+int32_t fn_window_samples(const fn_window *w, int32_t semantics, const double *t, int32_t n, double *out);
+/* argument checks of the MEX gateways, in their order, without running anything */
+int32_t fn_pw_validate(const fn_pw_params *p);
+int32_t fn_sa_validate(const fn_sa_params *p);
+
+/* ---- one-shot entry points (host buffers in and out) ------------------- */
+/* Replaces the body of gpu_bf_das_pw_rf.cu:110 after argument parsing:
+ * H2D, spectra, fused beamforming, D2H.  image: complex float [n_x][n_z];
+ * weights (may be NULL): sum of receive apodization weights of the last bin
+ * n_ub per voxel (the MEX's second output). */
+int32_t fn_das_pw(const fn_pw_params *p, const void *data_rf, int32_t rf_dtype, float *image,
+                  float *weights);
+/* Replaces gpu_bf_saft_rf.cu:107.  weights_tx / weights_rx may be NULL. */
+int32_t fn_das_sa(const fn_sa_params *p, const void *data_rf, int32_t rf_dtype, float *image,
+                  float *weights_tx, float *weights_rx);
+
+/* ---- plans: device-resident execution ---------------------------------- */
+/* A plan holds the validated configuration, the device copies of the grid and
+ * F-number tables and the spectra workspace for one shape.  fn_plan_execute
+ * takes device pointers and enqueues on `stream` (a hipStream_t, NULL = the
+ * plan's own stream) without host synchronisation. */
+typedef struct fn_plan fn_plan;
+typedef struct fn_plan_info {
+    int32_t N_dft, n_lb, n_ub, n_f;
+    int32_t n_tx;                 /* angles (PW) or elements (SA) */
+    int64_t n_voxels;
+    int64_t workspace_bytes;
+    double flops_per_frame;       /* 8 * n_f * n_voxels * n_elements * n_tx */
+    double useful_flops_per_frame;/* same, counting only valid (voxel, bin) pairs */
+    int32_t tile_voxels, bin_split, m_tiles, k_steps;
+} fn_plan_info;
+int32_t fn_plan_pw_create(const fn_pw_params *p, fn_plan **plan);
+int32_t fn_plan_sa_create(const fn_sa_params *p, fn_plan **plan);
+int32_t fn_plan_info_get(const fn_plan *plan, fn_plan_info *info);
+/* d_rf: device RF (rf_dtype); d_image: complex float [n_x][n_z]; weights may be NULL */
+int32_t fn_plan_execute(fn_plan *plan, const void *d_rf, int32_t rf_dtype, float *d_image,
+                        float *d_weights, float *d_weights_rx, void *stream);
+/* Per-stage timing with HIP events recorded on the execution stream around
+ * the spectra kernel and the beamforming kernel; totals since the last reset. */
+int32_t fn_plan_set_profiling(fn_plan *plan, int32_t enable);
+int32_t fn_plan_get_profile(fn_plan *plan, double *spectra_ms, double *beamform_ms, int32_t *n_runs);
+int32_t fn_plan_destroy(fn_plan *plan);
+
+/* ---- synthetic acquisitions (seeded; the reference ships no data) ------ */
+typedef struct fn_synth_params {
+    uint32_t struct_size;
+    uint64_t seed;                 /* scatterer seed; noise uses seed + 1000 */
+    int32_t n_points, n_speckle;   /* point scatterers (amplitude point_amplitude) + Gaussian speckle */
+    double point_amplitude;
+    double x_min, x_max, z_min, z_max;
+    int32_t n_samples, n_elements;
+    double f_s, element_pitch, c_0;
+    double f_c, fractional_bandwidth;  /* Gaussian-modulated pulse */
+    double snr_db;                 /* white noise relative to signal RMS; +inf: none */
+    int32_t mode;                  /* 0: plane waves (angles), 1: synthetic aperture */
+    int32_t n_angles;
+    const double *steering_angles;
+    int32_t on_device;             /* 1: rf is a device pointer, computed on the GPU */
+    int32_t index_gpu;
+} fn_synth_params;
+/* rf: float32 column-major [n_samples x n_elements x (n_angles | n_elements)] */
+int32_t fn_synth_rf(const fn_synth_params *p, float *rf);
+/* the seeded scatterer list itself: xs, zs, amps of length n_points + n_speckle */
+int32_t fn_synth_scatterers(const fn_synth_params *p, double *xs, double *zs, double *amps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FNUMBER_HIP_H */

@@ -1,0 +1,240 @@
+"""ctypes front end of the FP64 CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of bench.py as the checker / timed CPU baseline.
+The f_number_amd product never imports it.
+
+The oracle restates the reference's MATLAB CPU path (das_pw.m:302-485,
+das_sa.m:289-509).  Images are returned in the CPU path's scale
+(``2 * sum``, das_pw.m:384,467) as complex128 arrays shaped [n_z, n_x]
+(MATLAB's column-major [N_z x N_x] image).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+WIN = {"boxcar": 0, "hann": 1, "tukey": 2, "cosine": 3, "triangular": 4}
+SEM = {"cpu": 0, "mex": 1}
+NORM = {"off": 0, "on": 1, "windowed": 2}
+FN = {
+    "constant": 0,
+    "angle_lb": 1,
+    "distance_constant": 2,
+    "szabo": 3,
+    "directivity_soft": 4,
+    "directivity_rigid": 5,
+    "perrot": 6,
+    "distance_none": 7,
+}
+
+
+class _FNumber(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p0", C.c_double), ("p1", C.c_double), ("p2", C.c_double)]
+
+
+class _Args(C.Structure):
+    _fields_ = [
+        ("n_x", C.c_int), ("n_z", C.c_int),
+        ("x", C.POINTER(C.c_double)), ("z", C.POINTER(C.c_double)),
+        ("n_t", C.c_int), ("n_el", C.c_int), ("n_tx", C.c_int),
+        ("rf", C.c_void_p), ("rf_f32", C.c_int),
+        ("f_s", C.c_double), ("width", C.c_double), ("pitch", C.c_double), ("c_0", C.c_double),
+        ("f_lb", C.c_double), ("f_ub", C.c_double), ("index_t0", C.c_double),
+        ("angles", C.POINTER(C.c_double)),
+        ("win_rx", C.c_int), ("win_rx_param", C.c_double),
+        ("f_rx", _FNumber),
+        ("norm_rx", C.c_int), ("norm_rx_win", C.c_int), ("norm_rx_win_param", C.c_double),
+        ("win_tx", C.c_int), ("win_tx_param", C.c_double),
+        ("f_tx", _FNumber),
+        ("norm_tx", C.c_int), ("norm_tx_win", C.c_int), ("norm_tx_win_param", C.c_double),
+        ("semantics", C.c_int), ("spectra_mode", C.c_int), ("n_threads", C.c_int),
+        ("n_sel", C.c_int), ("sel_ix", C.POINTER(C.c_int)), ("sel_iz", C.POINTER(C.c_int)),
+        ("image_re", C.POINTER(C.c_double)), ("image_im", C.POINTER(C.c_double)),
+        ("N_dft", C.c_int), ("n_lb", C.c_int), ("n_ub", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build() -> str:
+    """Compile oracle.c with its Makefile (gcc, no GPU involved)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        d = C.POINTER(C.c_double)
+        i = C.POINTER(C.c_int)
+        L.or_grid.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_double, i, i, i]
+        L.or_fnumber_values.argtypes = [C.POINTER(_FNumber), d, C.c_int, d]
+        L.or_window.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int]
+        L.or_window.restype = C.c_double
+        L.or_spectra.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.c_int, d, d]
+        L.or_das_pw.argtypes = [C.POINTER(_Args)]
+        L.or_das_sa.argtypes = [C.POINTER(_Args)]
+        L.or_das_pw_c.argtypes = [C.POINTER(_Args), C.c_double, d]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def fnumber(kind: str, p0=0.0, p1=0.0, p2=0.0) -> _FNumber:
+    return _FNumber(FN[kind], float(p0), float(p1), float(p2))
+
+
+def grid(n_t, n_el, pitch, z_pad, f_s, c_0, f_lb, f_ub):
+    N, lb, ub = C.c_int(), C.c_int(), C.c_int()
+    rc = lib().or_grid(n_t, n_el, pitch, z_pad, f_s, c_0, f_lb, f_ub, C.byref(N), C.byref(lb), C.byref(ub))
+    if rc:
+        raise ValueError("empty frequency band")
+    return N.value, lb.value, ub.value
+
+
+def fnumber_values(F: _FNumber, r) -> np.ndarray:
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.empty_like(r)
+    rc = lib().or_fnumber_values(C.byref(F), _dp(r), r.size, _dp(out))
+    if rc:
+        raise ValueError(f"or_fnumber_values failed ({rc})")
+    return out
+
+
+def window(name: str, param: float, t, semantics: str = "cpu") -> np.ndarray:
+    t = np.atleast_1d(np.asarray(t, dtype=np.float64))
+    L = lib()
+    return np.array([L.or_window(WIN[name], float(param), float(v), SEM[semantics]) for v in t])
+
+
+def spectra(rf: np.ndarray, N: int, n_lb: int, n_ub: int, mode: int = 1, n_threads: int = 0) -> np.ndarray:
+    """rf: [n_ch, n_t] (C order, i.e. MATLAB column-major [n_t x n_ch]).  Returns [n_ch, n_f] complex."""
+    rf = np.ascontiguousarray(rf)
+    f32 = rf.dtype == np.float32
+    if not f32:
+        rf = np.ascontiguousarray(rf, dtype=np.float64)
+    n_ch, n_t = rf.shape
+    nf = n_ub - n_lb + 1
+    re = np.empty((n_ch, nf))
+    im = np.empty((n_ch, nf))
+    rc = lib().or_spectra(rf.ctypes.data, int(f32), n_t, n_ch, N, n_lb, n_ub, mode, n_threads, _dp(re), _dp(im))
+    if rc:
+        raise ValueError("or_spectra failed")
+    return re + 1j * im
+
+
+def _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics, spectra_mode, n_threads, sel):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    rf = np.ascontiguousarray(rf)
+    if rf.dtype != np.float32:
+        rf = np.ascontiguousarray(rf, dtype=np.float64)
+    a = _Args()
+    a.n_x, a.n_z = x.size, z.size
+    a.x, a.z = _dp(x), _dp(z)
+    a.rf = rf.ctypes.data
+    a.rf_f32 = int(rf.dtype == np.float32)
+    a.f_s, a.width, a.pitch, a.c_0 = f_s, width, pitch, c_0
+    a.f_lb, a.f_ub = f_bounds
+    a.index_t0 = index_t0
+    a.semantics = SEM[semantics]
+    a.spectra_mode = spectra_mode
+    a.n_threads = n_threads
+    keep = [x, z, rf]
+    if sel is not None:
+        ix = np.ascontiguousarray(sel[0], dtype=np.int32)
+        iz = np.ascontiguousarray(sel[1], dtype=np.int32)
+        a.n_sel = ix.size
+        a.sel_ix = ix.ctypes.data_as(C.POINTER(C.c_int))
+        a.sel_iz = iz.ctypes.data_as(C.POINTER(C.c_int))
+        keep += [ix, iz]
+        n_out = ix.size
+    else:
+        a.n_sel = 0
+        n_out = x.size * z.size
+    re = np.zeros(n_out)
+    im = np.zeros(n_out)
+    a.image_re, a.image_im = _dp(re), _dp(im)
+    keep += [re, im]
+    return a, keep, re, im
+
+
+def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
+           window=("tukey", 0.2), F=None, normalization="on", semantics="cpu",
+           spectra_mode=1, n_threads=0, sel=None, norm_window=("boxcar", 0.0),
+           c_tof=None, delay_offsets=None):
+    """rf: [n_tx, n_el, n_t] C-order (MATLAB [n_t x n_el x n_tx]).  Returns (image, info).
+
+    image is [n_z, n_x] complex (or [n_sel] with ``sel=(ix, iz)``), CPU scale."""
+    rf = np.asarray(rf)
+    if rf.ndim == 2:
+        rf = rf[None]
+    n_tx, n_el, n_t = rf.shape
+    angles = np.ascontiguousarray(np.atleast_1d(angles), dtype=np.float64)
+    assert angles.size == n_tx
+    a, keep, re, im = _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics,
+                            spectra_mode, n_threads, sel)
+    a.n_t, a.n_el, a.n_tx = n_t, n_el, n_tx
+    a.angles = _dp(angles)
+    a.win_rx, a.win_rx_param = WIN[window[0]], float(window[1])
+    a.f_rx = F if F is not None else fnumber("angle_lb", 60, 1.5, 10)
+    a.norm_rx = NORM[normalization]
+    a.norm_rx_win, a.norm_rx_win_param = WIN[norm_window[0]], float(norm_window[1])
+    if c_tof is None:
+        rc = lib().or_das_pw(C.byref(a))
+    else:
+        offs = None
+        if delay_offsets is not None:
+            offs = np.ascontiguousarray(delay_offsets, dtype=np.float64)
+            keep.append(offs)
+        rc = lib().or_das_pw_c(C.byref(a), float(c_tof), _dp(offs) if offs is not None else None)
+    if rc:
+        raise ValueError(f"or_das_pw failed ({rc})")
+    img = re + 1j * im
+    if sel is None:
+        img = img.reshape(np.asarray(x).size, np.asarray(z).size).T
+    return img, dict(N_dft=a.N_dft, n_lb=a.n_lb, n_ub=a.n_ub)
+
+
+def das_sa(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0=0.0,
+           window_tx=("tukey", 0.2), window_rx=("tukey", 0.2), F_tx=None, F_rx=None,
+           normalization_tx="on", normalization_rx="on", semantics="cpu", spectra_mode=1,
+           n_threads=0, sel=None, norm_window=("boxcar", 0.0)):
+    """rf: [n_tx, n_rx, n_t] C-order (MATLAB [n_t x n_rx x n_tx])."""
+    rf = np.asarray(rf)
+    n_tx, n_rx, n_t = rf.shape
+    assert n_tx == n_rx
+    a, keep, re, im = _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics,
+                            spectra_mode, n_threads, sel)
+    a.n_t, a.n_el, a.n_tx = n_t, n_rx, n_tx
+    a.win_tx, a.win_tx_param = WIN[window_tx[0]], float(window_tx[1])
+    a.win_rx, a.win_rx_param = WIN[window_rx[0]], float(window_rx[1])
+    a.f_tx = F_tx if F_tx is not None else fnumber("angle_lb", 60, 1.5, 10)
+    a.f_rx = F_rx if F_rx is not None else fnumber("angle_lb", 60, 1.5, 10)
+    a.norm_tx = NORM[normalization_tx]
+    a.norm_rx = NORM[normalization_rx]
+    a.norm_tx_win = a.norm_rx_win = WIN[norm_window[0]]
+    a.norm_tx_win_param = a.norm_rx_win_param = float(norm_window[1])
+    rc = lib().or_das_sa(C.byref(a))
+    if rc:
+        raise ValueError(f"or_das_sa failed ({rc})")
+    img = re + 1j * im
+    if sel is None:
+        img = img.reshape(np.asarray(x).size, np.asarray(z).size).T
+    return img, dict(N_dft=a.N_dft, n_lb=a.n_lb, n_ub=a.n_ub)

@@ -31,6 +31,11 @@ void clear_error() { g_last_error[0] = '\0'; }
 int frequency_grid(int n_t, int n_el, double pitch, double z_pad, double f_s, double c_0, double f_lb,
                    double f_ub, Grid *g)
 {
+    // gpu_bf_das_pw_rf.cu:301-303 precede the index check of :548-551
+    if (!(f_lb > 0)) return fail(FN_ERR_FREQUENCY_BOUND, "Lower frequency bound must be positive!");
+    if (!(f_ub > f_lb)) return fail(FN_ERR_FREQUENCY_BOUND, "Upper frequency bound must exceed the lower frequency bound!");
+    if (!(f_ub <= f_s / 2)) return fail(FN_ERR_FREQUENCY_BOUND, "Upper frequency bound must not exceed the Nyquist frequency!");
+    if (n_t < 1 || n_el < 1) return fail(FN_ERR_RF_DATA, "RF data must not be empty");
     const double M = (n_el - 1) / 2.0;
     const double span = ((n_el - 1) - M) * pitch - (0 - M) * pitch;
     const double pad = std::ceil((std::sqrt(span * span + z_pad * z_pad) - z_pad) * f_s / c_0);

@@ -1,0 +1,139 @@
+"""Pins of the FP64 beamformer oracle (das_pw.m:302-474, das_sa.m:289-498).
+
+1. oracle.c equals the statement-by-statement numpy transliteration of
+   das_pw.m (oracle/das_numpy.py, spectra from pocketfft) on small cases.
+2. F = 0, boxcar, normalization off reduces to plain Fourier DAS
+   sum_k sum_n 2 X_k(n) e^{j w_n tau_k}, brute-forced in long double.
+3. Linearity in the RF data, time-index shift = per-bin phase e^{j w_n i_t0 / f_s},
+   point-target focus at the scatterer, SA with a single active transmitter.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import das_numpy as DN
+
+FS, C0, P, W = 20.832e6, 1540.0, 0.3048e-3, 0.27e-3
+
+
+def _fn_np(kind):
+    if kind == "angle_lb":
+        return lambda r: DN.f_angle_lb(60, 1.5, 10, r)
+    return lambda r: np.full(np.shape(r), 1.5)
+
+
+@pytest.mark.parametrize("win", [("boxcar", 0.0), ("tukey", 0.5), ("hann", 0.0), ("triangular", 0.0)])
+@pytest.mark.parametrize("norm", ["on", "off"])
+@pytest.mark.parametrize("kind", ["angle_lb", "constant"])
+def test_oracle_equals_numpy_transliteration(oracle, win, norm, kind):
+    rng = np.random.default_rng(11)
+    n_el, n_t = 16, 256
+    x = np.linspace(-3e-3, 3e-3, 7)
+    z = np.linspace(5e-3, 12e-3, 9)
+    rf = rng.standard_normal((n_t, n_el)).astype(np.float32)
+    F = oracle.fnumber("angle_lb", 60, 1.5, 10) if kind == "angle_lb" else oracle.fnumber("constant", 1.5)
+    for ang, t0 in [(0.0, 0), (0.12, 3), (-0.2, 1)]:
+        ref = DN.das_pw(x, z, rf.astype(np.float64), FS, ang, W, P, C0, (2.25e6, 6.75e6), t0, win, _fn_np(kind), norm)
+        for mode in (0, 1):
+            img, _ = oracle.das_pw(x, z, rf.T[None], FS, [ang], W, P, C0, (2.25e6, 6.75e6), t0, win, F, norm,
+                                   spectra_mode=mode)
+            assert np.linalg.norm(img - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_plain_fourier_das_reduction(oracle):
+    """F = 0 (full aperture), boxcar, normalization off: the image is
+    sum_k sum_n 2 X_k(n) exp(j w_n tau_k), X from a long-double DFT."""
+    rng = np.random.default_rng(5)
+    n_el, n_t = 6, 120
+    x = np.array([-0.4e-3, 0.1e-3, 0.7e-3, 1.1e-3])
+    z = np.array([4e-3, 4.5e-3, 6e-3, 7.5e-3])
+    rf = rng.standard_normal((n_t, n_el))
+    ang, t0 = 0.15, 2
+    img, info = oracle.das_pw(x, z, rf.T[None], FS, [ang], W, P, C0, (3e6, 6e6), t0, ("boxcar", 0),
+                              oracle.fnumber("constant", 0.0), "off", spectra_mode=0)
+    N, lb, ub = info["N_dft"], info["n_lb"], info["n_ub"]
+    n = np.arange(lb, ub + 1, dtype=np.longdouble)
+    t = np.arange(n_t, dtype=np.longdouble)
+    ang_nt = 2 * np.pi * np.outer(n, t) / N
+    X = (np.cos(ang_nt) @ rf.astype(np.longdouble)) - 1j * (np.sin(ang_nt) @ rf.astype(np.longdouble))
+    M = (n_el - 1) / 2
+    xk = (np.arange(n_el) - M) * P
+    ex, ez = -math.sin(ang), math.cos(ang)
+    xref = math.copysign(1.0, ex) * xk[0]
+    ref = np.zeros((z.size, x.size), dtype=complex)
+    om = 2 * np.pi * FS * n / N
+    for ix, xv in enumerate(x):
+        for iz, zv in enumerate(z):
+            tau = (ex * (xv - xref) + ez * zv) / C0 + t0 / FS + np.sqrt((xk - xv) ** 2 + zv ** 2) / C0
+            ph = np.exp(1j * np.outer(om, tau).astype(np.float64))
+            ref[iz, ix] = np.sum(ph * 2 * X.astype(np.complex128))
+    assert np.linalg.norm(img - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_linearity_and_time_shift(oracle):
+    rng = np.random.default_rng(9)
+    n_el, n_t = 12, 200
+    x = np.linspace(-1e-3, 1e-3, 5)
+    z = np.linspace(4e-3, 9e-3, 6)
+    a, b = rng.standard_normal((2, n_t, n_el))
+    F = oracle.fnumber("angle_lb", 60, 1.5, 10)
+    args = (FS, [0.05], W, P, C0, (2e6, 7e6))
+    ia, _ = oracle.das_pw(x, z, a.T[None], *args, 0, ("tukey", 0.5), F, "on")
+    ib, _ = oracle.das_pw(x, z, b.T[None], *args, 0, ("tukey", 0.5), F, "on")
+    iab, _ = oracle.das_pw(x, z, (1.5 * a - 0.25 * b).T[None], *args, 0, ("tukey", 0.5), F, "on")
+    assert np.allclose(iab, 1.5 * ia - 0.25 * ib, rtol=1e-12, atol=1e-12 * np.abs(ia).max())
+    # a one-bin band: index_t0 = m multiplies the image by exp(j w_n m / f_s)
+    N = oracle.grid(n_t, n_el, P, z[0], FS, C0, 1e6, 9e6)[0]
+    n1 = round(5e6 * N / FS)
+    fb = ((n1 - 0.4) * FS / N, (n1 + 0.4) * FS / N)
+    N, lb, ub = oracle.grid(n_t, n_el, P, z[0], FS, C0, *fb)
+    assert lb == ub == n1
+    i0, _ = oracle.das_pw(x, z, a.T[None], FS, [0.05], W, P, C0, fb, 0, ("hann", 0), F, "on")
+    i3, _ = oracle.das_pw(x, z, a.T[None], FS, [0.05], W, P, C0, fb, 3, ("hann", 0), F, "on")
+    ph = np.exp(1j * 2 * np.pi * (FS * lb / N) * 3 / FS)
+    assert np.allclose(i3, ph * i0, rtol=1e-12, atol=1e-12 * np.abs(i0).max())
+
+
+def test_point_target_focus(oracle):
+    """A single point scatterer (no speckle, no noise) focuses at its own position."""
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.cpwc_small(), 41, 41, n_angles=3, n_samples=1024, n_speckle=0)
+    w.n_points, w.snr_db = 1, float("inf")
+    w.scatter_x, w.scatter_z = (1.0e-3, 1.0e-3 + 1e-12), (12e-3, 12e-3 + 1e-12)
+    w.positions_x = np.linspace(-1e-3, 3e-3, 41)
+    w.positions_z = np.linspace(10e-3, 14e-3, 41)
+    rf = w.rf_host()
+    img, _ = oracle.das_pw(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0, w.f_bounds,
+                           0, ("tukey", 0.5), oracle.fnumber("angle_lb", 60, 1.5, 10), "on")
+    iz, ix = np.unravel_index(np.argmax(np.abs(img)), img.shape)
+    assert abs(w.positions_x[ix] - 1e-3) <= 0.1e-3 + 1e-9
+    assert abs(w.positions_z[iz] - 12e-3) <= 0.1e-3 + 1e-9
+    # the focused echo is the pulse at t = 0: cos-modulated, so the peak is real positive
+    assert img[iz, ix].real > 0 and abs(img[iz, ix].imag) < 0.35 * abs(img[iz, ix])
+
+
+def test_saft_single_transmitter(oracle):
+    """SA with data from one transmitter only equals the brute-force double sum
+    sum_n a_tx a_rx e^{j w_n (tau_tx + tau_rx)} 2 X_{rx,tx}(n) for that transmitter (das_sa.m:485-491)."""
+    rng = np.random.default_rng(2)
+    n_el, n_t = 8, 160
+    x = np.linspace(-0.5e-3, 0.5e-3, 3)
+    z = np.linspace(4e-3, 6e-3, 3)
+    rf = np.zeros((n_el, n_el, n_t))
+    tx = 5
+    rf[tx] = rng.standard_normal((n_el, n_t))
+    F = oracle.fnumber("constant", 0.0)
+    img, info = oracle.das_sa(x, z, rf, FS, W, P, C0, (3e6, 6e6), 0, ("boxcar", 0), ("boxcar", 0), F, F,
+                              "off", "off", spectra_mode=0)
+    N, lb, ub = info["N_dft"], info["n_lb"], info["n_ub"]
+    X = np.fft.fft(rf[tx], N, axis=1)[:, lb:ub + 1]  # [rx, bins]
+    om = 2 * np.pi * FS * np.arange(lb, ub + 1) / N
+    xk = (np.arange(n_el) - (n_el - 1) / 2) * P
+    ref = np.zeros((z.size, x.size), dtype=complex)
+    for ix, xv in enumerate(x):
+        for iz, zv in enumerate(z):
+            tau = np.sqrt((xk - xv) ** 2 + zv ** 2) / C0
+            ref[iz, ix] = np.sum(np.exp(1j * np.outer(om, tau[tx] + tau)) * 2 * X.T)
+    assert np.linalg.norm(img - ref) <= 1e-12 * np.linalg.norm(ref)

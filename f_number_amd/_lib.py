@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfnumber_hip.so")
+# FN_LIB_PATH selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("FN_LIB_PATH") or os.path.join(_HERE, "libfnumber_hip.so")
 
 # enums (include/fnumber_hip.h)
 WINDOW = {"boxcar": 0, "hann": 1, "tukey": 2, "cosine": 3, "triangular": 4}

@@ -28,45 +28,49 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall"]
 
 
-def _units():
+def _units(extra):
     units = [
         ("fn_host.o", [HIPCC, "-x", "c++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "fn_host.cpp")]),
-        ("fn_core.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-c", os.path.join(CSRC, "fn_core.hip")]),
-        ("fn_synth.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-c", os.path.join(CSRC, "fn_synth.hip")]),
+        ("fn_core.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_core.hip")]),
+        ("fn_synth.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_synth.hip")]),
     ]
     for mode in (0, 1):
         for ki in (8, 16, 32):
             units.append((f"fn_beamform_m{mode}_k{ki}.o",
-                          [HIPCC, "-x", "hip"] + HIP_FLAGS +
+                          [HIPCC, "-x", "hip"] + HIP_FLAGS + extra +
                           [f"-DINST_MODE={mode}", f"-DINST_KI={ki}", "-c",
                            os.path.join(CSRC, "fn_beamform_inst.hip")]))
     return units
 
 
-def _run(name, cmd):
-    out = os.path.join(OBJ, name)
+def _run(obj_dir, name, cmd):
+    out = os.path.join(obj_dir, name)
     r = subprocess.run(cmd + ["-o", out], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile of {name} failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return out
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> str:
-    """Compile every translation unit for gfx950 and link the shared library."""
-    os.makedirs(OBJ, exist_ok=True)
-    units = _units()
+def build(jobs: int | None = None, verbose: bool = False, extra_flags=(), lib: str = LIB,
+          obj_dir: str = OBJ) -> str:
+    """Compile every translation unit for gfx950 and link the shared library.
+
+    extra_flags (e.g. -D tuning macros) and lib/obj_dir allow building variants
+    next to the default library."""
+    os.makedirs(obj_dir, exist_ok=True)
+    units = _units(list(extra_flags))
     jobs = jobs or min(len(units), os.cpu_count() or 4, 16)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda u: _run(*u), units))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda u: _run(obj_dir, *u), units))
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {lib}")
+    return lib
 
 
 if __name__ == "__main__":

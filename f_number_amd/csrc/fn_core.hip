@@ -54,7 +54,7 @@ struct fn_plan {
     fnk::BeamArgs args{};
     fnk::SpecArgs sargs{};
     size_t lds_beam = 0, lds_spec = 0;
-    double *d_px = nullptr, *d_pz = nullptr, *d_Frx = nullptr, *d_Ftx = nullptr, *d_txp = nullptr;
+    double *d_px = nullptr, *d_pz = nullptr, *d_Frx = nullptr, *d_Ftx = nullptr, *d_txp = nullptr, *d_sqf = nullptr;
     float2 *d_tw = nullptr, *d_X = nullptr;
     size_t x_bytes = 0;
     hipStream_t stream = nullptr;
@@ -147,7 +147,7 @@ struct Common {
 void destroy(fn_plan *p)
 {
     if (!p) return;
-    hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp);
+    hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp); hipFree(p->d_sqf);
     hipFree(p->d_tw); hipFree(p->d_X);
     for (auto &r : p->pending) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     for (auto &r : p->spare) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
@@ -179,7 +179,8 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     if (!P->KI) return fail(FN_ERR_UNSUPPORTED, "at most 128 array elements are supported (got %d)", c.n_el);
     P->MT_all = (c.n_tx + 15) / 16;
     if (P->mode == 0) {
-        const int groups = (P->MT_all + 7) / 8;
+        // at most 5 transmit tiles per group keep the LDS tile within 80 KiB
+        const int groups = (P->MT_all + 4) / 5;
         P->MT_tmpl = (P->MT_all + groups - 1) / groups;
         P->n_groups = (P->MT_all + P->MT_tmpl - 1) / P->MT_tmpl;
     } else {
@@ -204,6 +205,10 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     if ((s = upload(&P->d_pz, c.pz, c.n_z, "positions_z"))) return s;
     if ((s = upload(&P->d_Frx, Frx, g.n_f, "F-number"))) return s;
     if (Ftx && (s = upload(&P->d_Ftx, Ftx, g.n_f, "F-number tx"))) return s;
+    // sqrt(f_n) per bin for the depth clamp of the F-number
+    std::vector<double> sqf(g.n_f);
+    for (int b = 0; b < g.n_f; ++b) sqf[b] = std::sqrt(c.f_s * (double)(g.n_lb + b) / g.N);
+    if ((s = upload(&P->d_sqf, sqf.data(), sqf.size(), "sqrt(f)"))) return s;
     // twiddles e^{+j 2 pi m / N} rounded once from long double (conjugated in the kernel)
     std::vector<float2> tw(g.N);
     for (int m = 0; m < g.N; ++m) {
@@ -217,7 +222,7 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
 
     fnk::BeamArgs &A = P->args;
     A.X = P->d_X;
-    A.px = P->d_px; A.pz = P->d_pz; A.Frx = P->d_Frx; A.Ftx = P->d_Ftx; A.txp = P->d_txp;
+    A.px = P->d_px; A.pz = P->d_pz; A.Frx = P->d_Frx; A.Ftx = P->d_Ftx; A.txp = P->d_txp; A.sqrt_f = P->d_sqf;
     A.n_x = c.n_x; A.n_z = c.n_z; A.n_el = c.n_el; A.n_tx = c.n_tx;
     A.n_f = g.n_f; A.n_lb = g.n_lb; A.N = g.N;
     A.bin_chunk = chunk;

@@ -61,6 +61,7 @@ struct BeamArgs {
     const double *Frx;      // F-number per band bin (receive)
     const double *Ftx;      // F-number per band bin (transmit, SA)
     const double *txp;      // PW: per angle (e_x, e_z, x_ref)
+    const double *sqrt_f;   // sqrt(f_n) per band bin (depth clamp)
     float2 *image;
     float *weights, *weights_tx;
     int n_x, n_z, n_el, n_tx, n_f, n_lb, N;
@@ -98,12 +99,14 @@ struct Aperture {
     bool valid;             // non-empty aperture: the bin contributes
 };
 
-__device__ __forceinline__ Aperture make_aperture(double x, double z, double F, const BeamArgs &A, int support)
+// xp = x / pitch, zp = z / (2 pitch) are per-voxel constants; h / pitch = zp / F.
+__device__ __forceinline__ Aperture make_aperture(double x, double xp, double zp, double F, const BeamArgs &A,
+                                                  int support)
 {
     Aperture ap;
-    const double h = z / (2.0 * F);
-    const double ua = A.M + (x - h) / A.pitch;
-    const double ub = A.M + (x + h) / A.pitch;
+    const double hp = zp / F;  // h / pitch with h = z / (2F) (das_pw.m:387); F = 0 gives inf
+    const double ua = A.M + (xp - hp);
+    const double ub = A.M + (xp + hp);
     const double lbd = fmax(ceil(ua), 0.0);
     const double ubd = fmin(floor(ub), (double)(A.n_el - 1));
     ap.valid = lbd <= ubd;
@@ -112,8 +115,11 @@ __device__ __forceinline__ Aperture make_aperture(double x, double z, double F, 
     ap.inv_wl = ap.inv_wr = 0.0f;
     if (ap.valid) {
         const int lb = (int)lbd, ubi = (int)ubd;
-        const double wl = x - (((double)lb - A.M) * A.pitch - A.width / 2);
-        const double wr = (((double)ubi - A.M) * A.pitch + A.width / 2) - x;
+        // snapped half-widths (das_pw.m:407,411): x - (x_lb - w/2), (x_ub + w/2) - x.
+        // Correctly rounded reciprocals: with untruncated windows the normalising
+        // sum can nearly cancel and amplifies any error in 1/W.
+        const double wl = x - (((double)lb - A.M) * A.pitch - 0.5 * A.width);
+        const double wr = (((double)ubi - A.M) * A.pitch + 0.5 * A.width) - x;
         ap.inv_wl = (float)fabs(1.0 / wl);
         ap.inv_wr = (float)fabs(1.0 / wr);
         if (support == SUPPORT_OPEN_H) {
@@ -228,8 +234,17 @@ __global__ __launch_bounds__(256) void spectra_kernel(const T *__restrict__ rf, 
 // handles elements k = 4j + g.  M rows of the MFMA are transmit indices
 // (angles for PW, transmitting elements for SA), 16 per M-tile.
 // Lane l holds D[4g + r][l & 15] = Y(tx = 16 mt + 4g + r, voxel l & 15).
+// Register budget from the LDS footprint: tiles of up to 80 KiB let two blocks
+// share a CU, so registers are capped at two waves per SIMD; small tiles leave
+// the compiler free, larger ones allow one block per CU only.
+template <int MT, int KI>
+struct WavesPerEU {
+    static constexpr int bytes = MT * KI * 64 * 8;
+    static constexpr int value = (MT <= 2) ? 1 : (bytes <= 81920 ? 2 : 1);
+};
 template <int MODE, int MT, int KI, int WCR, int WCT>
-__global__ __launch_bounds__(256) void beamform_kernel(BeamArgs A)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<MT, KI>::value)))
+void beamform_kernel(BeamArgs A)
 {
     extern __shared__ float2 xs[];  // [MT][KI][64]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -240,6 +255,8 @@ __global__ __launch_bounds__(256) void beamform_kernel(BeamArgs A)
     const long vv = vok ? v : n_vox - 1;
     const int ix = (int)(vv / A.n_z), iz = (int)(vv - (long)ix * A.n_z);
     const double x = A.px[ix], z = A.pz[iz];
+    const double xp = x / A.pitch, zp = z / (2.0 * A.pitch);
+    const double clamp_z = sqrt(z / (2.88 * A.c0));  // sqrt(f z / (2.88 c)) = sqrt(f) clamp_z (das_pw.m:374)
     const int mt0 = blockIdx.z * A.tx_group;  // first M-tile of this transmit group
 
     // receive phase increments: tau_rx = sqrt(((x_k-x)/c)^2 + (z/c)^2) (das_pw.m:339-340,427)
@@ -299,63 +316,74 @@ __global__ __launch_bounds__(256) void beamform_kernel(BeamArgs A)
         }
         __syncthreads();
 
-        const double f_n = A.f_s * (double)n / A.N;  // das_pw.m:361
         double Fr = A.Frx[b];
-        if (A.clamp) Fr = fmin(Fr, sqrt(f_n * z / (2.88 * A.c0)));  // das_pw.m:374-377
-        const Aperture ap = make_aperture(x, z, Fr, A, A.support_rx);
+        if (A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_z);  // das_pw.m:374-377
+        const Aperture ap = make_aperture(x, xp, zp, Fr, A, A.support_rx);
 
-        floatx4 accR[MT], accI[MT];
+        // Gauss three-multiplication complex product on the matrix cores:
+        //   K1 = sum_k Br (Xr + Xi), K2 = sum_k Xr (Bi - Br), K3 = sum_k Xi (Br + Bi)
+        //   Re Y = K1 - K3, Im Y = K1 + K2
+        floatx4 K1[MT], K2[MT], K3[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
-            accR[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-            accI[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+            K1[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+            K2[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+            K3[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
         }
         float asum = 0.f;
         const int klo = ap.klo - g, khi = ap.khi - g;  // support in units of 4j
+        // recompute x_k - x per bin (one FMA each) instead of holding KI registers
+        float dgb = dg;
+        asm volatile("" : "+v"(dgb));
 #pragma unroll
         for (int j = 0; j < KI; ++j) {
-            const float d = fmaf((float)j, dstep, dg);
+            const float d = fmaf((float)j, dstep, dgb);
             const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
             float a = window_eval<WCR>(t, A.wrx);
             a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
             const uint32_t P = (uint32_t)n * Qrx[j];
             const float rv = rev_of(P);
             const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-            const float br = a * cs, bi = a * sn, nbi = -bi;
+            const float br = a * cs, bi = a * sn;
+            const float bd = bi - br, bp = br + bi;
             asum += a;
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const float2 xv = xs[(mt * KI + j) * 64 + lane];
-                accR[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, br, accR[mt], 0, 0, 0);
-                accI[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, bi, accI[mt], 0, 0, 0);
-                accR[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, nbi, accR[mt], 0, 0, 0);
-                accI[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, br, accI[mt], 0, 0, 0);
+                const float xsum = xv.x + xv.y;
+                K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xsum, br, K1[mt], 0, 0, 0);
+                K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, bd, K2[mt], 0, 0, 0);
+                K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, bp, K3[mt], 0, 0, 0);
             }
         }
         asum += __shfl_xor(asum, 16);
         asum += __shfl_xor(asum, 32);
+        floatx4 accR[MT], accI[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            accR[mt] = K1[mt] - K3[mt];
+            accI[mt] = K1[mt] + K2[mt];
+        }
 
         if (MODE == MODE_PW) {
             if (ap.valid) {
                 const float sc = A.norm_rx ? 1.0f / asum : 1.0f;
+                // padded transmit rows have zero spectra, so every row is summed
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int a = 16 * (mt0 + mt) + 4 * g + r;
-                        if (a < A.n_tx) {
-                            const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
-                            const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                            const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
-                            Ir += cs * yr - sn * yi;
-                            Ii += cs * yi + sn * yr;
-                        }
+                        const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
+                        const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                        const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
+                        Ir += cs * yr - sn * yi;
+                        Ii += cs * yi + sn * yr;
                     }
             }
         } else {
             // transmit aperture and apodization (das_sa.m:384-400, 438-455)
             const double Ft = A.Ftx[b];
-            const Aperture at = make_aperture(x, z, Ft, A, A.support_tx);
+            const Aperture at = make_aperture(x, xp, zp, Ft, A, A.support_tx);
             float wtx[MT][4];
             float tsum = 0.f;
 #pragma unroll

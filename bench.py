@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--arm", choices=["both", "hann", "boxcar"], default="both")
     ap.add_argument("--workload", choices=["cpwc_full", "cpwc_small", "saft_full"], default="cpwc_full")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -110,8 +112,8 @@ def run_arm(w, plan_kw, steps, warmup, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
-                                   "(sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "runs on one GPU only: --gpus N > 1 is not supported; a frame "
+                                   "shards by voxel tiles across GPUs without exchange"}))
         sys.exit(2)
     import torch
 
@@ -119,6 +121,13 @@ def main():
     from f_number_amd import synth
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize(dev)
+        del src, dst
     w = {"cpwc_full": synth.cpwc_full, "cpwc_small": synth.cpwc_small, "saft_full": synth.saft_full}[args.workload]()
     arms = {}
     if args.arm in ("both", "hann"):

@@ -1,0 +1,46 @@
+/* mex.h -- the subset of MATLAB's C MEX API used by the gateways in
+ * f_number_amd/csrc/mex/, implemented by fake_mex.cpp so the gateways can be
+ * exercised without MATLAB.  Names and signatures follow MATLAB's published API. */
+#ifndef FAKE_MEX_H
+#define FAKE_MEX_H
+#include <stddef.h>
+#include <stdbool.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+typedef struct mxArray_tag mxArray;
+typedef size_t mwSize;
+typedef size_t mwIndex;
+typedef enum { mxUNKNOWN_CLASS = 0, mxCELL_CLASS, mxSTRUCT_CLASS, mxLOGICAL_CLASS, mxCHAR_CLASS, mxVOID_CLASS,
+               mxDOUBLE_CLASS, mxSINGLE_CLASS, mxOBJECT_CLASS = 100 } mxClassID;
+typedef enum { mxREAL = 0, mxCOMPLEX } mxComplexity;
+
+bool mxIsDouble(const mxArray *a);
+bool mxIsComplex(const mxArray *a);
+bool mxIsEmpty(const mxArray *a);
+bool mxIsClass(const mxArray *a, const char *name);
+bool mxIsLogicalScalarTrue(const mxArray *a);
+mwSize mxGetNumberOfDimensions(const mxArray *a);
+const mwSize *mxGetDimensions(const mxArray *a);
+size_t mxGetM(const mxArray *a);
+size_t mxGetN(const mxArray *a);
+size_t mxGetNumberOfElements(const mxArray *a);
+double *mxGetPr(const mxArray *a);
+void *mxGetData(const mxArray *a);
+void *mxGetImagData(const mxArray *a);
+mxArray *mxGetProperty(const mxArray *a, mwIndex i, const char *name);
+int mxGetString(const mxArray *a, char *buf, mwSize len);
+mxArray *mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c);
+mxArray *mxCreateDoubleScalar(double v);
+mxArray *mxCreateString(const char *s);
+mxArray *mxCreateNumericMatrix(mwSize m, mwSize n, mxClassID cls, mxComplexity c);
+mxArray *mxDuplicateArray(const mxArray *a);
+void mxDestroyArray(mxArray *a);
+int mexCallMATLAB(int nlhs, mxArray *plhs[], int nrhs, mxArray *prhs[], const char *name);
+void mexErrMsgIdAndTxt(const char *id, const char *fmt, ...);
+int mexPrintf(const char *fmt, ...);
+#ifdef __cplusplus
+}
+void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]);
+#endif
+#endif

@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--arm", choices=["both", "hann", "boxcar"], default="both")
     ap.add_argument("--workload", choices=["cpwc_full", "cpwc_small", "saft_full"], default="cpwc_full")
+    ap.add_argument("--precisions", default="bf16x3,fp16x3,fp16",
+                    help="mixed-precision arms of the headline configuration (comma list, '' for none)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
@@ -74,7 +76,7 @@ def cpu_baseline(w, window_name, F_kind, F_params, n_threads):
     }
 
 
-def run_arm(w, plan_kw, steps, warmup, dev):
+def run_arm(w, plan_kw, steps, warmup, dev, keep_image=False):
     import torch
 
     import f_number_amd as fa
@@ -101,6 +103,7 @@ def run_arm(w, plan_kw, steps, warmup, dev):
     ok = bool(torch.isfinite(torch.view_as_real(img)).all().item())
     plan.close()
     return {
+        "image": img if keep_image else None,
         "ms_per_step": wall / steps * 1e3,
         "spectra_ms": spec_ms / max(n, 1),
         "beamform_ms": beam_ms / max(n, 1),
@@ -130,15 +133,28 @@ def main():
         del src, dst
     w = {"cpwc_full": synth.cpwc_full, "cpwc_small": synth.cpwc_small, "saft_full": synth.saft_full}[args.workload]()
     arms = {}
+    hann_kw = dict(window=fa.windows.hann(), F_number=fa.f_numbers.grating.angle_lb(60, 1.5))
+    precs = [p for p in args.precisions.split(",") if p]
     if args.arm in ("both", "hann"):
-        arms["hann_angle_lb"] = run_arm(w, dict(window=fa.windows.hann(),
-                                                F_number=fa.f_numbers.grating.angle_lb(60, 1.5)),
-                                        args.steps, args.warmup, dev)
+        arms["hann_angle_lb"] = run_arm(w, hann_kw, args.steps, args.warmup, dev, keep_image=bool(precs))
     if args.arm in ("both", "boxcar"):
         arms["boxcar_F1.5"] = run_arm(w, dict(window=fa.windows.boxcar(), F_number=fa.f_numbers.constant(1.5)),
                                       args.steps, args.warmup, dev)
     head_key = "hann_angle_lb" if "hann_angle_lb" in arms else "boxcar_F1.5"
     head = arms[head_key]
+    # mixed-precision contractions of the headline configuration: speed-up of
+    # the beamforming kernel and image error relative to the FP32 image
+    prec_arms = {}
+    if head_key == "hann_angle_lb":
+        ref_img = head["image"]
+        for p in precs:
+            r = run_arm(w, dict(hann_kw, precision=p), args.steps, args.warmup, dev, keep_image=True)
+            err = (torch.linalg.vector_norm(r["image"] - ref_img) / torch.linalg.vector_norm(ref_img)).item()
+            prec_arms[p] = {"ms_per_step": r["ms_per_step"], "beamform_ms": r["beamform_ms"],
+                            "prep_ms": r["spectra_ms"], "value": w.n_voxels / (r["ms_per_step"] / 1e3) / 1e6,
+                            "beamform_speedup_vs_fp32": head["beamform_ms"] / r["beamform_ms"],
+                            "rel_l2_vs_fp32": err, "finite": r["finite"]}
+            del r
     value = w.n_voxels / (head["ms_per_step"] / 1e3) / 1e6
     flops = head["info"]["flops_per_frame"]
     achieved = flops / (head["beamform_ms"] / 1e3) / 1e12
@@ -179,6 +195,7 @@ def main():
         "arms": {k: {"value": w.n_voxels / (v["ms_per_step"] / 1e3) / 1e6, "ms_per_step": v["ms_per_step"],
                      "beamform_ms": v["beamform_ms"], "spectra_ms": v["spectra_ms"], "finite": v["finite"]}
                  for k, v in arms.items()},
+        "precision_arms": prec_arms,
     }
     if not args.no_cpu_baseline:
         nthreads = min(os.cpu_count() or 1, 16)

@@ -36,10 +36,11 @@ def _units(extra):
     ]
     for mode in (0, 1):
         for ki in (8, 16, 32):
-            units.append((f"fn_beamform_m{mode}_k{ki}.o",
-                          [HIPCC, "-x", "hip"] + HIP_FLAGS + extra +
-                          [f"-DINST_MODE={mode}", f"-DINST_KI={ki}", "-c",
-                           os.path.join(CSRC, "fn_beamform_inst.hip")]))
+            for prec in (0, 1, 2, 3):
+                units.append((f"fn_beamform_m{mode}_k{ki}_p{prec}.o",
+                              [HIPCC, "-x", "hip"] + HIP_FLAGS + extra +
+                              [f"-DINST_MODE={mode}", f"-DINST_KI={ki}", f"-DINST_PREC={prec}", "-c",
+                               os.path.join(CSRC, "fn_beamform_inst.hip")]))
     return units
 
 

@@ -1,6 +1,6 @@
 // fn_beamform_inst.hip -- explicit instantiations of beamform_kernel for one
-// (mode, KI) pair, compiled once per pair (-DINST_MODE=0|1 -DINST_KI=8|16|32)
-// so the instantiation set builds in parallel.
+// (mode, KI, precision) triple, compiled once per triple (-DINST_MODE=0|1
+// -DINST_KI=8|16|32 -DINST_PREC=0..3) so the instantiation set builds in parallel.
 #include "fn_kernels.hip.h"
 
 #ifndef INST_MODE
@@ -9,16 +9,19 @@
 #ifndef INST_KI
 #error "INST_KI must be defined"
 #endif
+#ifndef INST_PREC
+#define INST_PREC 0
+#endif
 
-#define FN_CAT4(a, b, c, d) a##b##c##d
-#define FN_LAUNCHER(m, k) FN_CAT4(launch_beamform_m, m, _k, k)
+#define FN_CAT6(a, b, c, d, e, f) a##b##c##d##e##f
+#define FN_LAUNCHER(m, k, p) FN_CAT6(launch_beamform_m, m, _k, k, _p, p)
 
 namespace fnk {
 
 template <int MT, int WCR, int WCT>
 static hipError_t launch_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
-    auto kern = beamform_kernel<INST_MODE, MT, INST_KI, WCR, WCT>;
+    auto kern = beamform_kernel<INST_MODE, MT, INST_KI, WCR, WCT, INST_PREC>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -40,7 +43,7 @@ static hipError_t launch_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, const
     }
 }
 
-hipError_t FN_LAUNCHER(INST_MODE, INST_KI)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
+hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
                                             const BeamArgs &A)
 {
     (void)wct;
@@ -69,7 +72,7 @@ static hipError_t launch_wct(int wct, dim3 grid, size_t lds, hipStream_t s, cons
     }
 }
 
-hipError_t FN_LAUNCHER(INST_MODE, INST_KI)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
+hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
                                             const BeamArgs &A)
 {
     if (mt != SA_MT) return hipErrorInvalidValue;

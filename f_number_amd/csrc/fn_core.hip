@@ -15,12 +15,27 @@
 #include <vector>
 
 namespace fnk {
-hipError_t launch_beamform_m0_k8(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
-hipError_t launch_beamform_m0_k16(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
-hipError_t launch_beamform_m0_k32(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
-hipError_t launch_beamform_m1_k8(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
-hipError_t launch_beamform_m1_k16(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
-hipError_t launch_beamform_m1_k32(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+#define FN_DECL_LAUNCHERS(m, k)                                                                      \
+    hipError_t launch_beamform_m##m##_k##k##_p0(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
+    hipError_t launch_beamform_m##m##_k##k##_p1(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
+    hipError_t launch_beamform_m##m##_k##k##_p2(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
+    hipError_t launch_beamform_m##m##_k##k##_p3(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+FN_DECL_LAUNCHERS(0, 8)
+FN_DECL_LAUNCHERS(0, 16)
+FN_DECL_LAUNCHERS(0, 32)
+FN_DECL_LAUNCHERS(1, 8)
+FN_DECL_LAUNCHERS(1, 16)
+FN_DECL_LAUNCHERS(1, 32)
+#undef FN_DECL_LAUNCHERS
+typedef hipError_t (*BeamLauncher)(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+// [mode][KI = 8, 16, 32][precision]
+static const BeamLauncher kBeamLaunchers[2][3][4] = {
+    {{launch_beamform_m0_k8_p0, launch_beamform_m0_k8_p1, launch_beamform_m0_k8_p2, launch_beamform_m0_k8_p3},
+     {launch_beamform_m0_k16_p0, launch_beamform_m0_k16_p1, launch_beamform_m0_k16_p2, launch_beamform_m0_k16_p3},
+     {launch_beamform_m0_k32_p0, launch_beamform_m0_k32_p1, launch_beamform_m0_k32_p2, launch_beamform_m0_k32_p3}},
+    {{launch_beamform_m1_k8_p0, launch_beamform_m1_k8_p1, launch_beamform_m1_k8_p2, launch_beamform_m1_k8_p3},
+     {launch_beamform_m1_k16_p0, launch_beamform_m1_k16_p1, launch_beamform_m1_k16_p2, launch_beamform_m1_k16_p3},
+     {launch_beamform_m1_k32_p0, launch_beamform_m1_k32_p1, launch_beamform_m1_k32_p2, launch_beamform_m1_k32_p3}}};
 // zero fill used instead of hipMemsetAsync so the launch sequence stays
 // capturable into a graph
 __global__ void fill_zero_kernel(float *p, long n)
@@ -50,13 +65,16 @@ struct fn_plan {
     long n_vox = 0;
     int KI = 0, MT_all = 0, MT_tmpl = 0, n_groups = 1, bin_split = 1;
     int wcr = 0, wct = 0;
+    int precision = FN_PRECISION_FP32;
     int verbosity = 0;
     fnk::BeamArgs args{};
     fnk::SpecArgs sargs{};
     size_t lds_beam = 0, lds_spec = 0;
     double *d_px = nullptr, *d_pz = nullptr, *d_Frx = nullptr, *d_Ftx = nullptr, *d_txp = nullptr, *d_sqf = nullptr;
     float2 *d_tw = nullptr, *d_X = nullptr;
-    size_t x_bytes = 0;
+    void *d_Xlp = nullptr;    // 16-bit spectra planes (mixed precision)
+    float *d_xscale = nullptr;
+    size_t x_bytes = 0, xlp_bytes = 0;
     hipStream_t stream = nullptr;
     int profiling = 0;
     std::vector<Run> pending, spare;
@@ -148,7 +166,7 @@ void destroy(fn_plan *p)
 {
     if (!p) return;
     hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp); hipFree(p->d_sqf);
-    hipFree(p->d_tw); hipFree(p->d_X);
+    hipFree(p->d_tw); hipFree(p->d_X); hipFree(p->d_Xlp); hipFree(p->d_xscale);
     for (auto &r : p->pending) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     for (auto &r : p->spare) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     if (p->stream) hipStreamDestroy(p->stream);
@@ -166,8 +184,8 @@ int upload(T **dst, const T *src, size_t n, const char *what)
 // shared part of plan creation
 int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double *Frx, const double *Ftx)
 {
-    if (c.precision != FN_PRECISION_FP32)
-        return fail(FN_ERR_UNSUPPORTED, "precision %d is not available in this build (FP32 only)", c.precision);
+    if (c.precision < FN_PRECISION_FP32 || c.precision > FN_PRECISION_FP16)
+        return fail(FN_ERR_ARGUMENT, "unknown precision %d", c.precision);
     int s = select_device(c.index_gpu);
     if (s) return s;
     P->device = c.index_gpu;
@@ -175,6 +193,7 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     P->n_x = c.n_x; P->n_z = c.n_z; P->n_t = c.n_t; P->n_el = c.n_el; P->n_tx = c.n_tx;
     P->n_vox = (long)c.n_x * c.n_z;
     P->verbosity = c.verbosity;
+    P->precision = c.precision;
     P->KI = ki_of(c.n_el);
     if (!P->KI) return fail(FN_ERR_UNSUPPORTED, "at most 128 array elements are supported (got %d)", c.n_el);
     P->MT_all = (c.n_tx + 15) / 16;
@@ -187,7 +206,11 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
         P->MT_tmpl = P->KI / 4;
         P->n_groups = 1;
     }
-    P->lds_beam = (size_t)P->MT_tmpl * P->KI * 64 * sizeof(float2);
+    // staged spectra per bin: FP32 interleaved complex (8 bytes per value) or
+    // 16-bit planes (Xr, Xi and, for the split forms, their residuals)
+    const int planes = c.precision == FN_PRECISION_FP16 ? 2 : 4;
+    const size_t step_bytes = c.precision == FN_PRECISION_FP32 ? 64 * sizeof(float2) : (size_t)planes * 64 * 2;
+    P->lds_beam = (size_t)P->MT_tmpl * P->KI * step_bytes;
     P->lds_spec = (size_t)g.N * sizeof(float2) + 64 * 65 * sizeof(float);
     if (P->lds_beam > kMaxLds) return fail(FN_ERR_UNSUPPORTED, "transmit tile exceeds LDS");
     if (P->lds_spec > kMaxLds) return fail(FN_ERR_UNSUPPORTED, "DFT order %d too large for the LDS twiddle table", g.N);
@@ -219,9 +242,16 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     P->x_bytes = (size_t)g.n_f * P->MT_all * P->KI * 64 * sizeof(float2);
     HIP_CHECK(hipMalloc(&P->d_X, P->x_bytes), "spectra workspace");
     HIP_CHECK(hipMemset(P->d_X, 0, P->x_bytes), "spectra workspace");  // padding rows stay zero
+    if (c.precision != FN_PRECISION_FP32) {
+        P->xlp_bytes = (size_t)g.n_f * planes * P->MT_all * P->KI * 64 * 2;
+        HIP_CHECK(hipMalloc(&P->d_Xlp, P->xlp_bytes), "16-bit spectra planes");
+        HIP_CHECK(hipMalloc(&P->d_xscale, (size_t)g.n_f * sizeof(float)), "spectra scales");
+    }
 
     fnk::BeamArgs &A = P->args;
     A.X = P->d_X;
+    A.Xlp = P->d_Xlp;
+    A.xscale = P->d_xscale;
     A.px = P->d_px; A.pz = P->d_pz; A.Frx = P->d_Frx; A.Ftx = P->d_Ftx; A.txp = P->d_txp; A.sqrt_f = P->d_sqf;
     A.n_x = c.n_x; A.n_z = c.n_z; A.n_el = c.n_el; A.n_tx = c.n_tx;
     A.n_f = g.n_f; A.n_lb = g.n_lb; A.N = g.N;
@@ -265,16 +295,8 @@ double useful_fraction(const fnh::Grid &g, const double *px, int nx, const doubl
 
 hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipStream_t s)
 {
-    const int ki = P->KI, mt = P->MT_tmpl;
-    const size_t lds = P->lds_beam;
-    if (P->mode == 0) {
-        if (ki == 8) return fnk::launch_beamform_m0_k8(mt, P->wcr, P->wct, grid, lds, s, A);
-        if (ki == 16) return fnk::launch_beamform_m0_k16(mt, P->wcr, P->wct, grid, lds, s, A);
-        return fnk::launch_beamform_m0_k32(mt, P->wcr, P->wct, grid, lds, s, A);
-    }
-    if (ki == 8) return fnk::launch_beamform_m1_k8(mt, P->wcr, P->wct, grid, lds, s, A);
-    if (ki == 16) return fnk::launch_beamform_m1_k16(mt, P->wcr, P->wct, grid, lds, s, A);
-    return fnk::launch_beamform_m1_k32(mt, P->wcr, P->wct, grid, lds, s, A);
+    const int kidx = P->KI == 8 ? 0 : (P->KI == 16 ? 1 : 2);
+    return fnk::kBeamLaunchers[P->mode][kidx][P->precision](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
 }
 
 int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w, float *d_w_rx, hipStream_t user)
@@ -319,6 +341,21 @@ int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w,
                            reinterpret_cast<const double *>(d_rf), P->d_tw, P->d_X, S);
     }
     HIP_CHECK(hipGetLastError(), "spectra_kernel");
+    if (P->precision != FN_PRECISION_FP32) {
+        const dim3 cg((unsigned)S.n_f), cb(256);
+        switch (P->precision) {
+        case FN_PRECISION_BF16X3:
+            hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_BF16X3>, cg, cb, 0, s, P->d_X, P->d_Xlp, P->d_xscale, S.MT, S.KI);
+            break;
+        case FN_PRECISION_FP16X3:
+            hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_FP16X3>, cg, cb, 0, s, P->d_X, P->d_Xlp, P->d_xscale, S.MT, S.KI);
+            break;
+        default:
+            hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_FP16>, cg, cb, 0, s, P->d_X, P->d_Xlp, P->d_xscale, S.MT, S.KI);
+            break;
+        }
+        HIP_CHECK(hipGetLastError(), "lowp_planes_kernel");
+    }
     if (P->profiling) HIP_CHECK(hipEventRecord(run.e1, s), "hipEventRecord");
     const long tiles = (P->n_vox + 63) / 64;
     hipError_t e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups), A, s);

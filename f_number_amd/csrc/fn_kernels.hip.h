@@ -57,6 +57,8 @@ enum { SUPPORT_OPEN_H = 0,   // boxcar, CPU: |x_k - x| / h < 1 (das_pw.m:441)
 
 struct BeamArgs {
     const float2 *X;        // spectra in operand order [n_f][MT][KI][64]
+    const void *Xlp;        // 16-bit spectra planes (mixed precision) [n_f][planes][MT][KI/8][64][8]
+    const float *xscale;    // per-bin scale of the 16-bit planes
     const double *px, *pz;  // voxel positions
     const double *Frx;      // F-number per band bin (receive)
     const double *Ftx;      // F-number per band bin (transmit, SA)
@@ -230,10 +232,77 @@ __global__ __launch_bounds__(256) void spectra_kernel(const T *__restrict__ rf, 
 // -------------------------------------------------------------------------
 // fused beamforming
 // -------------------------------------------------------------------------
-// block: 256 threads = 4 waves; wave = 16 voxels (MFMA N), lane group g = lane/16
-// handles elements k = 4j + g.  M rows of the MFMA are transmit indices
-// (angles for PW, transmitting elements for SA), 16 per M-tile.
-// Lane l holds D[4g + r][l & 15] = Y(tx = 16 mt + 4g + r, voxel l & 15).
+// block: 256 threads = 4 waves; wave = 16 voxels (MFMA N); lane group g = lane/16.
+// M rows of the MFMA are transmit indices (angles for PW, transmitting elements
+// for SA), 16 per M-tile; lane l holds D[4g + r][l & 15] = Y(tx = 16 mt + 4g + r,
+// voxel l & 15).  K is the receive element:
+//   FP32  (v_mfma_f32_16x16x4f32):      lane group g supplies element 4j + g at step j
+//   16-bit (v_mfma_f32_16x16x32_{bf16,f16}): lane group g supplies elements
+//          32c + 8g + s, s = 0..7, of chunk c (8 packed values per lane)
+// Both give every lane KI elements j = 0..KI-1 (kelem()).
+enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP16X3 = 2, PREC_FP16 = 3 };
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int PREC>
+struct Prec {
+    static constexpr int planes = (PREC == PREC_FP16) ? 2 : 4;  // Xr, Xi (+ lo parts)
+    static constexpr bool split = (PREC == PREC_BF16X3 || PREC == PREC_FP16X3);
+};
+
+template <int PREC>
+__device__ __forceinline__ int kelem(int j, int g)
+{
+    return PREC == PREC_FP32 ? 4 * j + g : 32 * (j >> 3) + 8 * g + (j & 7);
+}
+
+// bytes of one bin of staged spectra per transmit M-tile and element step
+template <int PREC>
+__host__ __device__ constexpr int stage_bytes_per_tile_step()
+{
+    return PREC == PREC_FP32 ? 64 * 8 : Prec<PREC>::planes * 64 * 2;
+}
+
+template <int PREC>
+__device__ __forceinline__ void pack8(const float *v, u32x4 &hi, u32x4 &lo)
+{
+    if (PREC == PREC_BF16X3) {
+        bf16x8 h, l;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            h[s] = (__bf16)v[s];
+            l[s] = (__bf16)(v[s] - (float)h[s]);
+        }
+        hi = __builtin_bit_cast(u32x4, h);
+        lo = __builtin_bit_cast(u32x4, l);
+    } else {
+        f16x8 h, l;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            h[s] = (_Float16)v[s];
+            if (PREC == PREC_FP16X3) l[s] = (_Float16)(v[s] - (float)h[s]);
+        }
+        hi = __builtin_bit_cast(u32x4, h);
+        if (PREC == PREC_FP16X3) lo = __builtin_bit_cast(u32x4, l);
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ floatx4 mfma16(const u32x4 &a, const u32x4 &b, const floatx4 &c)
+{
+    if (PREC == PREC_BF16X3)
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ u32x4 neg16(const u32x4 &v)
+{
+    const unsigned m = 0x80008000u;
+    return u32x4{v[0] ^ m, v[1] ^ m, v[2] ^ m, v[3] ^ m};
+}
+
 // Register budget from the LDS footprint: tiles of up to 80 KiB let two blocks
 // share a CU, so registers are capped at two waves per SIMD; small tiles leave
 // the compiler free, larger ones allow one block per CU only.
@@ -242,11 +311,12 @@ struct WavesPerEU {
     static constexpr int bytes = MT * KI * 64 * 8;
     static constexpr int value = (MT <= 2) ? 1 : (bytes <= 81920 ? 2 : 1);
 };
-template <int MODE, int MT, int KI, int WCR, int WCT>
+
+template <int MODE, int MT, int KI, int WCR, int WCT, int PREC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<MT, KI>::value)))
 void beamform_kernel(BeamArgs A)
 {
-    extern __shared__ float2 xs[];  // [MT][KI][64]
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c = lane & 15;
     const long n_vox = (long)A.n_x * A.n_z;
@@ -265,7 +335,7 @@ void beamform_kernel(BeamArgs A)
     uint32_t Qrx[KI];
 #pragma unroll
     for (int j = 0; j < KI; ++j) {
-        const int k = 4 * j + g;
+        const int k = kelem<PREC>(j, g);
         const double lat = (((double)k - A.M) * A.pitch - x) / A.c0;
         const double tau = sqrt(lat * lat + zc * zc) + t_half;
         Qrx[j] = frac32(A.f_s * tau / A.N);
@@ -278,41 +348,53 @@ void beamform_kernel(BeamArgs A)
         for (int r = 0; r < 4; ++r) {
             const int a = 16 * (mt0 + mt) + 4 * g + r;
             double tau = 0.0;
-            if (MODE == MODE_PW) {
-                // das_pw.m:336: (e_x (x - x_ref) + e_z z) / c + i_t0 / f_s
-                if (a < A.n_tx) {
+            if (a < A.n_tx) {
+                if (MODE == MODE_PW) {
+                    // das_pw.m:336: (e_x (x - x_ref) + e_z z) / c + i_t0 / f_s
                     const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
                     tau = (ex * (x - xr) + ez * z) / A.c0 + A.t0 / A.f_s;
-                }
-            } else {
-                if (a < A.n_tx) {
+                } else {
                     const double lat = (((double)a - A.M) * A.pitch - x) / A.c0;
                     tau = sqrt(lat * lat + zc * zc) + t_half;
                 }
             }
             Qtx[mt][r] = frac32(A.f_s * tau / A.N);
         }
-    const float dg = (float)(((double)g - A.M) * A.pitch - x);   // x_k - x for j = 0
-    const float dstep = (float)(4.0 * A.pitch);
+    const int k0 = kelem<PREC>(0, g);
+    const float dg = (float)(((double)k0 - A.M) * A.pitch - x);  // x_k - x of element j = 0
+    const float pf = (float)A.pitch;
     const float dg_tx = (float)((4.0 * g - A.M) * A.pitch - x);  // SA transmit rows, mt = r = 0
 
     float Ir = 0.f, Ii = 0.f, w_last = 0.f, wtx_last = 0.f;
     const int b_begin = blockIdx.y * A.bin_chunk;
     const int b_end = min(b_begin + A.bin_chunk, A.n_f);
-    const int n_f4 = MT * KI * 32;  // float4 per bin
-    const float4 *Xg = reinterpret_cast<const float4 *>(A.X);
-    const int MT_total = (A.n_tx + 15) >> 4;
-    const int MT_all = MT_total;  // stride of the operand layout
+    const int MT_all = (A.n_tx + 15) >> 4;  // stride of the operand layout
+    // staged bytes per bin: this group's tiles of every plane
+    constexpr int tile_bytes = stage_bytes_per_tile_step<PREC>() * KI;  // one M-tile, all elements
+    constexpr int n_f4 = MT * tile_bytes / 16;
+    const int mt_here = min(MT, MT_all - mt0);
 
     for (int b = b_begin; b < b_end; ++b) {
         const int n = A.n_lb + b;
         __syncthreads();
         {
-            // stage this group's M-tiles of X_n into LDS
-            const float4 *src = Xg + ((size_t)b * MT_all + mt0) * KI * 32;
-            float4 *dst = reinterpret_cast<float4 *>(xs);
-            const int n4 = min(MT, MT_all - mt0) * KI * 32;
-            for (int e = threadIdx.x; e < n_f4; e += 256) dst[e] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 *dst = reinterpret_cast<float4 *>(smem_raw);
+            if (PREC == PREC_FP32) {
+                const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)b * MT_all + mt0) * (tile_bytes / 16);
+                const int n4 = mt_here * (tile_bytes / 16);
+                for (int e = threadIdx.x; e < n_f4; e += 256) dst[e] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                // planes [p][mt][c][64][8] per bin; copy tiles mt0.. of each plane
+                constexpr int pl = Prec<PREC>::planes;
+                constexpr int tile4 = tile_bytes / pl / 16;  // float4 per tile per plane
+                const float4 *src = reinterpret_cast<const float4 *>(A.Xlp) + (size_t)b * pl * MT_all * tile4;
+                for (int e = threadIdx.x; e < n_f4; e += 256) {
+                    const int p = e / (MT * tile4), rem = e - p * (MT * tile4);
+                    const int mt = rem / tile4;
+                    dst[e] = (mt < mt_here) ? src[((size_t)p * MT_all + mt0 + mt) * tile4 + (rem - mt * tile4)]
+                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            }
         }
         __syncthreads();
 
@@ -320,54 +402,113 @@ void beamform_kernel(BeamArgs A)
         if (A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_z);  // das_pw.m:374-377
         const Aperture ap = make_aperture(x, xp, zp, Fr, A, A.support_rx);
 
-        // Gauss three-multiplication complex product on the matrix cores:
-        //   K1 = sum_k Br (Xr + Xi), K2 = sum_k Xr (Bi - Br), K3 = sum_k Xi (Br + Bi)
-        //   Re Y = K1 - K3, Im Y = K1 + K2
-        floatx4 K1[MT], K2[MT], K3[MT];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            K1[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-            K2[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-            K3[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-        }
+        floatx4 accR[MT], accI[MT];
         float asum = 0.f;
-        const int klo = ap.klo - g, khi = ap.khi - g;  // support in units of 4j
         // recompute x_k - x per bin (one FMA each) instead of holding KI registers
         float dgb = dg;
         asm volatile("" : "+v"(dgb));
-#pragma unroll
-        for (int j = 0; j < KI; ++j) {
-            const float d = fmaf((float)j, dstep, dgb);
-            const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
-            float a = window_eval<WCR>(t, A.wrx);
-            a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
-            const uint32_t P = (uint32_t)n * Qrx[j];
-            const float rv = rev_of(P);
-            const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-            const float br = a * cs, bi = a * sn;
-            const float bd = bi - br, bp = br + bi;
-            asum += a;
+        if (PREC == PREC_FP32) {
+            // Gauss three-multiplication complex product on the matrix cores:
+            //   K1 = sum_k Br (Xr + Xi), K2 = sum_k Xr (Bi - Br), K3 = sum_k Xi (Br + Bi)
+            //   Re Y = K1 - K3, Im Y = K1 + K2
+            const float2 *xs = reinterpret_cast<const float2 *>(smem_raw);
+            floatx4 K1[MT], K2[MT], K3[MT];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                const float2 xv = xs[(mt * KI + j) * 64 + lane];
-                const float xsum = xv.x + xv.y;
-                K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xsum, br, K1[mt], 0, 0, 0);
-                K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, bd, K2[mt], 0, 0, 0);
-                K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, bp, K3[mt], 0, 0, 0);
+                K1[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                K2[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                K3[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
+            const int klo = ap.klo - g, khi = ap.khi - g;  // support in units of 4j
+            const float dstep = 4.0f * pf;
+            for (int j = 0; j < KI; ++j) {
+                const float d = fmaf((float)j, dstep, dgb);
+                const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
+                float a = window_eval<WCR>(t, A.wrx);
+                a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
+                const float rv = rev_of((uint32_t)n * Qrx[j]);
+                const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                const float br = a * cs, bi = a * sn;
+                const float bd = bi - br, bp = br + bi;
+                asum += a;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const float2 xv = xs[(mt * KI + j) * 64 + lane];
+                    const float xsum = xv.x + xv.y;
+                    K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xsum, br, K1[mt], 0, 0, 0);
+                    K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, bd, K2[mt], 0, 0, 0);
+                    K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, bp, K3[mt], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                accR[mt] = K1[mt] - K3[mt];
+                accI[mt] = K1[mt] + K2[mt];
+            }
+        } else {
+            // 16-bit operands, FP32 accumulation: Re Y = Xr Br - Xi Bi, Im Y = Xr Bi + Xi Br;
+            // split forms add hi*lo + lo*hi passes (x = hi + lo)
+            constexpr int KC = KI / 8;
+            constexpr int pl = Prec<PREC>::planes;
+            const u32x4 *xq = reinterpret_cast<const u32x4 *>(smem_raw);  // [p][mt][c][64]
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                accR[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                accI[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll
+            for (int cc = 0; cc < KC; ++cc) {
+                float br[8], bi[8];
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const int j = 8 * cc + s;
+                    const int k = kelem<PREC>(j, g);
+                    const float d = fmaf((float)(k - k0), pf, dgb);
+                    const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
+                    float a = window_eval<WCR>(t, A.wrx);
+                    a = (k >= ap.klo && k <= ap.khi) ? a : 0.f;
+                    const float rv = rev_of((uint32_t)n * Qrx[j]);
+                    const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                    br[s] = a * cs;
+                    bi[s] = a * sn;
+                    asum += a;
+                }
+                u32x4 brh, brl, bih, bil;
+                pack8<PREC>(br, brh, brl);
+                pack8<PREC>(bi, bih, bil);
+                const u32x4 nbih = neg16(bih);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const u32x4 xrh = xq[((0 * MT + mt) * KC + cc) * 64 + lane];
+                    const u32x4 xih = xq[((1 * MT + mt) * KC + cc) * 64 + lane];
+                    accR[mt] = mfma16<PREC>(xrh, brh, accR[mt]);
+                    accR[mt] = mfma16<PREC>(xih, nbih, accR[mt]);
+                    accI[mt] = mfma16<PREC>(xrh, bih, accI[mt]);
+                    accI[mt] = mfma16<PREC>(xih, brh, accI[mt]);
+                    if (Prec<PREC>::split) {
+                        const u32x4 xrl = xq[((2 * MT + mt) * KC + cc) * 64 + lane];
+                        const u32x4 xil = xq[((3 * MT + mt) * KC + cc) * 64 + lane];
+                        const u32x4 nbil = neg16(bil);
+                        accR[mt] = mfma16<PREC>(xrh, brl, accR[mt]);
+                        accR[mt] = mfma16<PREC>(xrl, brh, accR[mt]);
+                        accR[mt] = mfma16<PREC>(xih, nbil, accR[mt]);
+                        accR[mt] = mfma16<PREC>(xil, nbih, accR[mt]);
+                        accI[mt] = mfma16<PREC>(xrh, bil, accI[mt]);
+                        accI[mt] = mfma16<PREC>(xrl, bih, accI[mt]);
+                        accI[mt] = mfma16<PREC>(xih, brl, accI[mt]);
+                        accI[mt] = mfma16<PREC>(xil, brh, accI[mt]);
+                    }
+                }
+            }
+            (void)pl;
         }
         asum += __shfl_xor(asum, 16);
         asum += __shfl_xor(asum, 32);
-        floatx4 accR[MT], accI[MT];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            accR[mt] = K1[mt] - K3[mt];
-            accI[mt] = K1[mt] + K2[mt];
-        }
+        const float xscale = (PREC == PREC_FP32) ? 1.0f : A.xscale[b];  // spectra were stored / xscale
 
         if (MODE == MODE_PW) {
             if (ap.valid) {
-                const float sc = A.norm_rx ? 1.0f / asum : 1.0f;
+                const float sc = (A.norm_rx ? 1.0f / asum : 1.0f) * xscale;
                 // padded transmit rows have zero spectra, so every row is summed
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
@@ -391,7 +532,7 @@ void beamform_kernel(BeamArgs A)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int i = 16 * (mt0 + mt) + 4 * g + r;
-                    const float d = fmaf((float)(16 * (mt0 + mt) + r), (float)A.pitch, dg_tx);
+                    const float d = fmaf((float)(16 * (mt0 + mt) + r), pf, dg_tx);
                     const float t = fabsf(d) * (d < 0.f ? at.inv_wl : at.inv_wr);
                     float w = window_eval<WCT>(t, A.wtx);
                     w = (i >= at.klo && i <= at.khi) ? w : 0.f;
@@ -401,18 +542,20 @@ void beamform_kernel(BeamArgs A)
             tsum += __shfl_xor(tsum, 16);
             tsum += __shfl_xor(tsum, 32);
             if (ap.valid && at.valid) {
-                const float sc = (A.norm_rx ? 1.0f / asum : 1.0f) * (A.norm_tx ? 1.0f / tsum : 1.0f);
+                const float sc = (A.norm_rx ? 1.0f / asum : 1.0f) * (A.norm_tx ? 1.0f / tsum : 1.0f) * xscale;
+                float pr = 0.f, pi = 0.f;
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
                         const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                        const float w = wtx[mt][r] * sc;
-                        const float yr = accR[mt][r] * w, yi = accI[mt][r] * w;
-                        Ir += cs * yr - sn * yi;
-                        Ii += cs * yi + sn * yr;
+                        const float yr = accR[mt][r] * wtx[mt][r], yi = accI[mt][r] * wtx[mt][r];
+                        pr += cs * yr - sn * yi;
+                        pi += cs * yi + sn * yr;
                     }
+                Ir = fmaf(sc, pr, Ir);
+                Ii = fmaf(sc, pi, Ii);
             }
             if (b == A.n_f - 1) wtx_last = tsum;
         }
@@ -433,6 +576,72 @@ void beamform_kernel(BeamArgs A)
         if (b_end == A.n_f && blockIdx.z == 0) {
             if (A.weights) A.weights[v] = w_last;
             if (MODE == MODE_SA && A.weights_tx) A.weights_tx[v] = wtx_last;
+        }
+    }
+}
+
+// -------------------------------------------------------------------------
+// 16-bit spectra planes for the mixed-precision contraction
+// -------------------------------------------------------------------------
+// One block per bin: reads the FP32 operand-ordered spectra of the bin, picks a
+// power-of-two scale (16-bit float variants: largest component in [2^14, 2^15)),
+// and writes planes [p][mt][c][64][8] (p: Xr hi, Xi hi, Xr lo, Xi lo) with
+// lane (a & 15) + 16 ((k & 31) >> 3), slot k & 7, chunk k >> 5.
+template <int PREC>
+__global__ __launch_bounds__(256) void lowp_planes_kernel(const float2 *__restrict__ X, void *__restrict__ Xlp,
+                                                          float *__restrict__ xscale, int MT_all, int KI)
+{
+    __shared__ float red[256];
+    const int b = blockIdx.x;
+    const int n_pairs = MT_all * KI * 64;
+    const float2 *Xb = X + (size_t)b * n_pairs;
+    float m = 0.f;
+    for (int e = threadIdx.x; e < n_pairs; e += 256) {
+        const float2 v = Xb[e];
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const float mx = red[0];
+    float scale = 1.0f;
+    if (PREC != PREC_BF16X3 && mx > 0.f) {
+        int e2;
+        frexpf(mx, &e2);              // mx = f 2^e2, f in [0.5, 1)
+        scale = ldexpf(1.0f, e2 - 15);  // mx / scale in [2^14, 2^15)
+    }
+    if (threadIdx.x == 0) xscale[b] = scale;
+    const float inv = 1.0f / scale;
+    constexpr int pl = Prec<PREC>::planes;
+    const int KC = KI / 8;
+    const int plane_elems = MT_all * KC * 64 * 8;  // 16-bit values per plane
+    unsigned short *out = reinterpret_cast<unsigned short *>(Xlp) + (size_t)b * pl * plane_elems;
+    for (int o = threadIdx.x; o < plane_elems; o += 256) {
+        const int s = o & 7, ln = (o >> 3) & 63, q = o >> 9;  // q = mt * KC + c
+        const int mt = q / KC, cc = q - mt * KC;
+        const int row = ln & 15, gg = ln >> 4;
+        const int k = 32 * cc + 8 * gg + s;
+        const float2 v = Xb[(mt * KI + (k >> 2)) * 64 + row + 16 * (k & 3)];
+        const float vr = v.x * inv, vi = v.y * inv;
+        if (PREC == PREC_BF16X3) {
+            const __bf16 rh = (__bf16)vr, ih = (__bf16)vi;
+            const __bf16 rl = (__bf16)(vr - (float)rh), il = (__bf16)(vi - (float)ih);
+            out[o] = __builtin_bit_cast(unsigned short, rh);
+            out[plane_elems + o] = __builtin_bit_cast(unsigned short, ih);
+            out[2 * plane_elems + o] = __builtin_bit_cast(unsigned short, rl);
+            out[3 * plane_elems + o] = __builtin_bit_cast(unsigned short, il);
+        } else {
+            const _Float16 rh = (_Float16)vr, ih = (_Float16)vi;
+            out[o] = __builtin_bit_cast(unsigned short, rh);
+            out[plane_elems + o] = __builtin_bit_cast(unsigned short, ih);
+            if (PREC == PREC_FP16X3) {
+                const _Float16 rl = (_Float16)(vr - (float)rh), il = (_Float16)(vi - (float)ih);
+                out[2 * plane_elems + o] = __builtin_bit_cast(unsigned short, rl);
+                out[3 * plane_elems + o] = __builtin_bit_cast(unsigned short, il);
+            }
         }
     }
 }

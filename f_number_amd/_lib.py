@@ -114,6 +114,12 @@ PROTOTYPES = {
     "fn_plan_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
     "fn_plan_get_profile": (C.c_int32, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32)]),
     "fn_plan_destroy": (C.c_int32, [C.c_void_p]),
+    "fn_plan_set_tof": (C.c_int32, [C.c_void_p, C.c_double, _dp]),
+    "fn_plan_execute_tangent": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]),
+    "fn_plan_adjoint": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fn_plan_vjp": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p]),
     "fn_synth_rf": (C.c_int32, [C.POINTER(SynthParams), C.c_void_p]),
     "fn_synth_scatterers": (C.c_int32, [C.POINTER(SynthParams), _dp, _dp, _dp]),
 }
@@ -129,6 +135,14 @@ def load():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m f_number_amd.build` "
                 "(the HIP library is required; there is no fallback path)")
+        # One HIP runtime per process: torch ships its own libamdhip64 (SONAME
+        # libamdhip64.so.7, but its modules link the unversioned name).  Loaded
+        # first, it satisfies this library's libamdhip64.so.7 dependency; loaded
+        # after /opt/rocm's copy it would be a second runtime that sees no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in PROTOTYPES.items():
             fn = getattr(L, name)

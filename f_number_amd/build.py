@@ -33,6 +33,7 @@ def _units(extra):
         ("fn_host.o", [HIPCC, "-x", "c++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "fn_host.cpp")]),
         ("fn_core.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_core.hip")]),
         ("fn_synth.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_synth.hip")]),
+        ("fn_adjoint.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_adjoint.hip")]),
     ]
     for mode in (0, 1):
         for ki in (8, 16, 32):

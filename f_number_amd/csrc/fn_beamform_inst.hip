@@ -43,6 +43,43 @@ static hipError_t launch_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, const
     }
 }
 
+#if INST_PREC == 0
+// tangent (dI / dc_tof) variant: groups of at most two transmit tiles
+#define FN_TAN_LAUNCHER(k) FN_CAT4(launch_beamform_tan, _k, k, )
+#define FN_CAT4(a, b, c, d) a##b##c##d
+template <int MT, int WCR>
+static hipError_t launch_tan_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    auto kern = beamform_kernel<0, MT, INST_KI, WCR, WC_RANGE, 0, true>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
+    return hipGetLastError();
+}
+
+hipError_t FN_TAN_LAUNCHER(INST_KI)(int mt, int wcr, dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    if (mt == 1) {
+        switch (wcr) {
+        case WC_RANGE: return launch_tan_one<1, WC_RANGE>(grid, lds, s, A);
+        case WC_LINEAR: return launch_tan_one<1, WC_LINEAR>(grid, lds, s, A);
+        default: return launch_tan_one<1, WC_COS>(grid, lds, s, A);
+        }
+    }
+    if (mt == 2) {
+        switch (wcr) {
+        case WC_RANGE: return launch_tan_one<2, WC_RANGE>(grid, lds, s, A);
+        case WC_LINEAR: return launch_tan_one<2, WC_LINEAR>(grid, lds, s, A);
+        default: return launch_tan_one<2, WC_COS>(grid, lds, s, A);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+#endif
+
 hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
                                             const BeamArgs &A)
 {

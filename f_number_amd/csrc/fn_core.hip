@@ -27,6 +27,17 @@ FN_DECL_LAUNCHERS(1, 8)
 FN_DECL_LAUNCHERS(1, 16)
 FN_DECL_LAUNCHERS(1, 32)
 #undef FN_DECL_LAUNCHERS
+hipError_t launch_beamform_tan_k8(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_tan_k16(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_tan_k32(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_adjoint(int wcr, dim3 grid, hipStream_t s, const AdjArgs &S);
+hipError_t launch_adjoint_reduce(const float *P, float2 *Z, int n_vr, int n_f, int n_at, int n_tx, int n_el,
+                                 hipStream_t s);
+hipError_t launch_adjoint_idft(const float2 *Z, const float2 *tw, float *out, int n_t, int n_ch, int N, int n_lb,
+                               int n_f, hipStream_t s);
+hipError_t launch_vjp_delay(const float2 *X, const float2 *Z, double *out, int n_f, int n_tx, int n_el, int MT, int KI,
+                            int n_lb, double w_unit, hipStream_t s);
+hipError_t launch_dot_re(const float2 *g, const float2 *d, long n, double *part, int n_part, double *out, hipStream_t s);
 typedef hipError_t (*BeamLauncher)(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 // [mode][KI = 8, 16, 32][precision]
 static const BeamLauncher kBeamLaunchers[2][3][4] = {
@@ -74,6 +85,13 @@ struct fn_plan {
     float2 *d_tw = nullptr, *d_X = nullptr;
     void *d_Xlp = nullptr;    // 16-bit spectra planes (mixed precision)
     float *d_xscale = nullptr;
+    double *d_delay = nullptr;  // per-element delay offsets (zeros unless set)
+    // adjoint / vector-Jacobian workspace (allocated on first use)
+    float *d_P = nullptr;       // voxel-range partials of the voxel contraction
+    float2 *d_Z = nullptr;      // reduced contraction [n_f][n_tx][n_el]
+    double *d_part = nullptr;   // reduction partials
+    int adj_vr = 0, adj_at = 0;
+    long adj_vpr = 0;
     size_t x_bytes = 0, xlp_bytes = 0;
     hipStream_t stream = nullptr;
     int profiling = 0;
@@ -167,6 +185,7 @@ void destroy(fn_plan *p)
     if (!p) return;
     hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp); hipFree(p->d_sqf);
     hipFree(p->d_tw); hipFree(p->d_X); hipFree(p->d_Xlp); hipFree(p->d_xscale);
+    hipFree(p->d_delay); hipFree(p->d_P); hipFree(p->d_Z); hipFree(p->d_part);
     for (auto &r : p->pending) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     for (auto &r : p->spare) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     if (p->stream) hipStreamDestroy(p->stream);
@@ -248,8 +267,14 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
         HIP_CHECK(hipMalloc(&P->d_xscale, (size_t)g.n_f * sizeof(float)), "spectra scales");
     }
 
+    HIP_CHECK(hipMalloc(&P->d_delay, (size_t)c.n_el * sizeof(double)), "delay offsets");
+    HIP_CHECK(hipMemset(P->d_delay, 0, (size_t)c.n_el * sizeof(double)), "delay offsets");
+
     fnk::BeamArgs &A = P->args;
     A.X = P->d_X;
+    A.delay = P->d_delay;
+    A.ctof = c.c0;
+    A.dimage = nullptr;
     A.Xlp = P->d_Xlp;
     A.xscale = P->d_xscale;
     A.px = P->d_px; A.pz = P->d_pz; A.Frx = P->d_Frx; A.Ftx = P->d_Ftx; A.txp = P->d_txp; A.sqrt_f = P->d_sqf;
@@ -299,12 +324,34 @@ hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipS
     return fnk::kBeamLaunchers[P->mode][kidx][P->precision](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
 }
 
+int launch_spectra(fn_plan *P, const void *d_rf, int dtype, hipStream_t s)
+{
+    const fnk::SpecArgs &S = P->sargs;
+    dim3 sg((unsigned)((S.n_f + 63) / 64), (unsigned)((S.n_ch + 63) / 64));
+    if (dtype == FN_DTYPE_F32) {
+        hipLaunchKernelGGL(fnk::spectra_kernel<float>, sg, dim3(256), P->lds_spec, s,
+                           reinterpret_cast<const float *>(d_rf), P->d_tw, P->d_X, S);
+    } else {
+        hipLaunchKernelGGL(fnk::spectra_kernel<double>, sg, dim3(256), P->lds_spec, s,
+                           reinterpret_cast<const double *>(d_rf), P->d_tw, P->d_X, S);
+    }
+    HIP_CHECK(hipGetLastError(), "spectra_kernel");
+    return FN_OK;
+}
+
+int fill_zero(float *p, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(fnk::fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    HIP_CHECK(hipGetLastError(), "fill_zero_kernel");
+    return FN_OK;
+}
+
 int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w, float *d_w_rx, hipStream_t user)
 {
     if (!P || !d_rf || !d_image) return fail(FN_ERR_ARGUMENT, "null argument");
     if (dtype != FN_DTYPE_F32 && dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
     HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
-    hipStream_t s = user ? user : P->stream;
+    hipStream_t s = user;  // NULL: the default stream
     fnk::BeamArgs A = P->args;
     A.image = reinterpret_cast<float2 *>(d_image);
     if (P->mode == 0) {
@@ -332,15 +379,7 @@ int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w,
     }
     if (P->profiling) HIP_CHECK(hipEventRecord(run.e0, s), "hipEventRecord");
     const fnk::SpecArgs &S = P->sargs;
-    dim3 sg((unsigned)((S.n_f + 63) / 64), (unsigned)((S.n_ch + 63) / 64));
-    if (dtype == FN_DTYPE_F32) {
-        hipLaunchKernelGGL(fnk::spectra_kernel<float>, sg, dim3(256), P->lds_spec, s,
-                           reinterpret_cast<const float *>(d_rf), P->d_tw, P->d_X, S);
-    } else {
-        hipLaunchKernelGGL(fnk::spectra_kernel<double>, sg, dim3(256), P->lds_spec, s,
-                           reinterpret_cast<const double *>(d_rf), P->d_tw, P->d_X, S);
-    }
-    HIP_CHECK(hipGetLastError(), "spectra_kernel");
+    if (int st = launch_spectra(P, d_rf, dtype, s)) return st;
     if (P->precision != FN_PRECISION_FP32) {
         const dim3 cg((unsigned)S.n_f), cb(256);
         switch (P->precision) {
@@ -452,6 +491,134 @@ int sa_plan(const fn_sa_params *p, fn_plan **out)
 
 size_t dtype_size(int dtype) { return dtype == FN_DTYPE_F64 ? 8 : 4; }
 
+// ---- differentiation form ----------------------------------------------
+
+int set_tof(fn_plan *P, double c_tof, const double *delays)
+{
+    if (!P) return fail(FN_ERR_ARGUMENT, "null plan");
+    if (!(c_tof == c_tof) || c_tof > DBL_MAX) return fail(FN_ERR_ARGUMENT, "c_tof must be finite");
+    std::vector<double> d(P->n_el, 0.0);
+    if (delays)
+        for (int k = 0; k < P->n_el; ++k) {
+            if (!(std::fabs(delays[k]) <= 1.0)) return fail(FN_ERR_ARGUMENT, "delay offset %d must be finite and |d| <= 1 s", k);
+            d[k] = delays[k];
+        }
+    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
+    HIP_CHECK(hipStreamSynchronize(P->stream), "hipStreamSynchronize");
+    HIP_CHECK(hipMemcpy(P->d_delay, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice), "delay offsets");
+    P->args.ctof = c_tof > 0 ? c_tof : P->args.c0;
+    return FN_OK;
+}
+
+int check_pw_fp32(const fn_plan *P, const char *what)
+{
+    if (!P) return fail(FN_ERR_ARGUMENT, "null plan");
+    if (P->mode != 0) return fail(FN_ERR_UNSUPPORTED, "%s: plane-wave plans only", what);
+    if (P->n_el > 128) return fail(FN_ERR_UNSUPPORTED, "%s: at most 128 elements", what);
+    return FN_OK;
+}
+
+int execute_tangent(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_dimage, float *d_w,
+                    hipStream_t user)
+{
+    if (int st = check_pw_fp32(P, "tangent")) return st;
+    if (!d_rf || !d_image || !d_dimage) return fail(FN_ERR_ARGUMENT, "null argument");
+    if (dtype != FN_DTYPE_F32 && dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
+    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
+    hipStream_t s = user;  // NULL: the default stream
+    if (int st = launch_spectra(P, d_rf, dtype, s)) return st;
+    // groups of at most two transmit tiles keep the tangent accumulators in registers
+    const int mt = std::min(P->MT_all, 2);
+    const int groups = (P->MT_all + mt - 1) / mt;
+    fnk::BeamArgs A = P->args;
+    A.image = reinterpret_cast<float2 *>(d_image);
+    A.dimage = d_dimage;
+    A.weights = d_w;
+    A.weights_tx = nullptr;
+    A.tx_group = mt;
+    A.atomic_out = (P->bin_split > 1 || groups > 1) ? 1 : 0;
+    if (A.atomic_out) {
+        if (int st = fill_zero(d_image, 2 * P->n_vox, s)) return st;
+        if (int st = fill_zero(d_dimage, 2 * P->n_vox, s)) return st;
+    }
+    const size_t lds = (size_t)mt * P->KI * 64 * sizeof(float2);
+    const dim3 grid((unsigned)((P->n_vox + 63) / 64), (unsigned)P->bin_split, (unsigned)groups);
+    hipError_t e = P->KI == 8 ? fnk::launch_beamform_tan_k8(mt, P->wcr, grid, lds, s, A)
+                 : P->KI == 16 ? fnk::launch_beamform_tan_k16(mt, P->wcr, grid, lds, s, A)
+                               : fnk::launch_beamform_tan_k32(mt, P->wcr, grid, lds, s, A);
+    if (e != hipSuccess) return hip_fail(e, "beamform tangent kernel");
+    return FN_OK;
+}
+
+// voxel contraction Z_n(k,a) = s sum_v y conj(T B) / w into P->d_Z
+int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
+{
+    const int n_f = P->g.n_f;
+    const int n_chunks = (n_f + 7) / 8;  // bins per block of the contraction kernel
+    if (!P->d_Z) {
+        P->adj_at = (P->n_tx + 15) / 16;
+        const long groups16 = (P->n_vox + 15) / 16;
+        // enough blocks for 256 CUs (one block per CU at a time), bounded by the voxel groups
+        long vr = (512 + (long)n_chunks * P->adj_at - 1) / ((long)n_chunks * P->adj_at);
+        vr = std::max(1L, std::min(vr, (groups16 + 3) / 4));
+        const long groups_per_range = (groups16 + vr - 1) / vr;
+        P->adj_vpr = groups_per_range * 16;
+        P->adj_vr = (int)((groups16 + groups_per_range - 1) / groups_per_range);
+        const size_t p_floats = (size_t)P->adj_vr * n_f * P->adj_at * 2 * 128 * 16;
+        HIP_CHECK(hipMalloc(&P->d_P, p_floats * sizeof(float)), "adjoint partials");
+        HIP_CHECK(hipMalloc(&P->d_Z, (size_t)n_f * P->n_tx * P->n_el * sizeof(float2)), "adjoint contraction");
+        HIP_CHECK(hipMalloc(&P->d_part, 1024 * sizeof(double)), "reduction partials");
+    }
+    fnk::AdjArgs S{};
+    S.B = P->args;
+    S.y = reinterpret_cast<const float2 *>(d_y);
+    S.P = P->d_P;
+    S.vox_per_range = P->adj_vpr;
+    S.n_at = P->adj_at;
+    const dim3 grid((unsigned)P->adj_vr, (unsigned)n_chunks, (unsigned)P->adj_at);
+    hipError_t e = fnk::launch_adjoint(P->wcr, grid, s, S);
+    if (e != hipSuccess) return hip_fail(e, "adjoint_kernel");
+    e = fnk::launch_adjoint_reduce(P->d_P, P->d_Z, P->adj_vr, n_f, P->adj_at, P->n_tx, P->n_el, s);
+    if (e != hipSuccess) return hip_fail(e, "adjoint_reduce_kernel");
+    return FN_OK;
+}
+
+int adjoint(fn_plan *P, const float *d_y, float *d_rf_adj, hipStream_t user)
+{
+    if (int st = check_pw_fp32(P, "adjoint")) return st;
+    if (!d_y || !d_rf_adj) return fail(FN_ERR_ARGUMENT, "null argument");
+    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
+    hipStream_t s = user;  // NULL: the default stream
+    if (int st = contract_adjoint(P, d_y, s)) return st;
+    hipError_t e = fnk::launch_adjoint_idft(P->d_Z, P->d_tw, d_rf_adj, P->n_t, P->n_tx * P->n_el, P->g.N, P->g.n_lb,
+                                            P->g.n_f, s);
+    if (e != hipSuccess) return hip_fail(e, "adjoint_idft_kernel");
+    return FN_OK;
+}
+
+int vjp(fn_plan *P, const void *d_rf, int dtype, const float *d_g, const float *d_dimage, double *d_vjp_delay,
+        double *d_vjp_c, hipStream_t user)
+{
+    if (int st = check_pw_fp32(P, "vjp")) return st;
+    if (!d_rf || !d_g || !d_vjp_delay) return fail(FN_ERR_ARGUMENT, "null argument");
+    if (dtype != FN_DTYPE_F32 && dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
+    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
+    hipStream_t s = user;  // NULL: the default stream
+    if (int st = launch_spectra(P, d_rf, dtype, s)) return st;
+    if (int st = contract_adjoint(P, d_g, s)) return st;
+    const double w_unit = 2.0 * 3.14159265358979323846 * P->args.f_s / P->g.N;
+    hipError_t e = fnk::launch_vjp_delay(P->d_X, P->d_Z, d_vjp_delay, P->g.n_f, P->n_tx, P->n_el, P->MT_all, P->KI,
+                                         P->g.n_lb, w_unit, s);
+    if (e != hipSuccess) return hip_fail(e, "vjp_delay_kernel");
+    if (d_dimage && d_vjp_c) {
+        const int n_part = (int)std::min<long>(1024, (P->n_vox + 255) / 256);
+        e = fnk::launch_dot_re(reinterpret_cast<const float2 *>(d_g), reinterpret_cast<const float2 *>(d_dimage),
+                               P->n_vox, P->d_part, n_part, d_vjp_c, s);
+        if (e != hipSuccess) return hip_fail(e, "dot_re_kernel");
+    }
+    return FN_OK;
+}
+
 // one-shot: H2D, execute, D2H (gpu_bf_das_pw_rf.cu:730-1122)
 int one_shot(fn_plan *P, const void *rf, int dtype, float *image, float *w0, float *w1)
 {
@@ -467,7 +634,7 @@ int one_shot(fn_plan *P, const void *rf, int dtype, float *image, float *w0, flo
     if (e == hipSuccess && w1) e = hipMalloc(&d_w1, w_bytes);
     if (e == hipSuccess) e = hipMemcpyAsync(d_rf, rf, rf_bytes, hipMemcpyHostToDevice, P->stream);
     if (e != hipSuccess) s = hip_fail(e, "one-shot buffers");
-    if (!s) s = execute(P, d_rf, dtype, d_img, d_w0, d_w1, nullptr);
+    if (!s) s = execute(P, d_rf, dtype, d_img, d_w0, d_w1, P->stream);
     if (!s) {
         e = hipMemcpyAsync(image, d_img, img_bytes, hipMemcpyDeviceToHost, P->stream);
         if (e == hipSuccess && w0) e = hipMemcpyAsync(w0, d_w0, w_bytes, hipMemcpyDeviceToHost, P->stream);
@@ -530,6 +697,32 @@ int32_t fn_plan_execute(fn_plan *P, const void *d_rf, int32_t dtype, float *d_im
 {
     fnh::clear_error();
     return execute(P, d_rf, dtype, d_image, d_w, d_w_rx, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t fn_plan_set_tof(fn_plan *P, double c_tof, const double *delay_offsets)
+{
+    fnh::clear_error();
+    return set_tof(P, c_tof, delay_offsets);
+}
+
+int32_t fn_plan_execute_tangent(fn_plan *P, const void *d_rf, int32_t dtype, float *d_image, float *d_dimage_dc,
+                                float *d_w, void *stream)
+{
+    fnh::clear_error();
+    return execute_tangent(P, d_rf, dtype, d_image, d_dimage_dc, d_w, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t fn_plan_adjoint(fn_plan *P, const float *d_y, float *d_rf_adj, void *stream)
+{
+    fnh::clear_error();
+    return adjoint(P, d_y, d_rf_adj, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t fn_plan_vjp(fn_plan *P, const void *d_rf, int32_t dtype, const float *d_g, const float *d_dimage_dc,
+                    double *d_vjp_delay, double *d_vjp_c, void *stream)
+{
+    fnh::clear_error();
+    return vjp(P, d_rf, dtype, d_g, d_dimage_dc, d_vjp_delay, d_vjp_c, reinterpret_cast<hipStream_t>(stream));
 }
 
 int32_t fn_plan_set_profiling(fn_plan *P, int32_t enable)

@@ -74,8 +74,20 @@ struct BeamArgs {
     int norm_rx, norm_tx;
     int atomic_out;
     double M, pitch, width, c0, f_s, t0;
+    double ctof;            // speed of sound of the time of flight (c0 unless set)
+    const double *delay;    // per-element delay offsets (s), [n_el]
+    float *dimage;          // tangent: dI / dc_tof, interleaved complex [n_vox]
     WinCoef wrx, wtx;
     float out_scale;
+};
+
+// adjoint voxel contraction (fn_adjoint.hip)
+struct AdjArgs {
+    BeamArgs B;
+    const float2 *y;        // cotangent image [n_vox]
+    float *P;               // partials [n_vr][n_f][n_at][2][128][16]
+    long vox_per_range;     // multiple of 16
+    int n_at;               // angle tiles of 16
 };
 
 struct SpecArgs {
@@ -312,10 +324,14 @@ struct WavesPerEU {
     static constexpr int value = (MT <= 2) ? 1 : (bytes <= 81920 ? 2 : 1);
 };
 
-template <int MODE, int MT, int KI, int WCR, int WCT, int PREC>
+// TAN (plane waves, FP32): also accumulates the tangent dI/dc_tof.  Geometric
+// delays scale as 1/c, so d/dc e^{j w tau} = j w (-(tau_tx,g + tau_rx,g) / c) e^{j w tau}:
+// a second contraction with B' = B tau_rx,g and the row factor tau_tx,g.
+template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<MT, KI>::value)))
 void beamform_kernel(BeamArgs A)
 {
+    static_assert(!TAN || (MODE == MODE_PW && PREC == PREC_FP32), "tangent: plane waves, FP32");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c = lane & 15;
@@ -330,35 +346,43 @@ void beamform_kernel(BeamArgs A)
     const int mt0 = blockIdx.z * A.tx_group;  // first M-tile of this transmit group
 
     // receive phase increments: tau_rx = sqrt(((x_k-x)/c)^2 + (z/c)^2) (das_pw.m:339-340,427)
-    const double zc = z / A.c0;
+    // plus the element's delay offset
+    const double zc = z / A.ctof;
     const double t_half = (MODE == MODE_SA) ? 0.5 * A.t0 / A.f_s : 0.0;  // das_sa.m:431
     uint32_t Qrx[KI];
+    float Trx[TAN ? KI : 1];  // tangent: geometric receive time (s)
 #pragma unroll
     for (int j = 0; j < KI; ++j) {
         const int k = kelem<PREC>(j, g);
-        const double lat = (((double)k - A.M) * A.pitch - x) / A.c0;
-        const double tau = sqrt(lat * lat + zc * zc) + t_half;
+        const double lat = (((double)k - A.M) * A.pitch - x) / A.ctof;
+        const double tg = sqrt(lat * lat + zc * zc);
+        const double tau = tg + t_half + (k < A.n_el ? A.delay[k] : 0.0);
         Qrx[j] = frac32(A.f_s * tau / A.N);
+        if (TAN) Trx[TAN ? j : 0] = (float)tg;
     }
     // transmit phase increments for the rows this lane holds
     uint32_t Qtx[MT][4];
+    float Ttx[TAN ? MT : 1][4];  // tangent: geometric transmit time (s)
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int a = 16 * (mt0 + mt) + 4 * g + r;
-            double tau = 0.0;
+            double tau = 0.0, tg = 0.0;
             if (a < A.n_tx) {
                 if (MODE == MODE_PW) {
                     // das_pw.m:336: (e_x (x - x_ref) + e_z z) / c + i_t0 / f_s
                     const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
-                    tau = (ex * (x - xr) + ez * z) / A.c0 + A.t0 / A.f_s;
+                    tg = (ex * (x - xr) + ez * z) / A.ctof;
+                    tau = tg + A.t0 / A.f_s;
                 } else {
-                    const double lat = (((double)a - A.M) * A.pitch - x) / A.c0;
-                    tau = sqrt(lat * lat + zc * zc) + t_half;
+                    // transmitting element a: its geometric delay and its delay offset
+                    const double lat = (((double)a - A.M) * A.pitch - x) / A.ctof;
+                    tau = sqrt(lat * lat + zc * zc) + t_half + A.delay[a];
                 }
             }
             Qtx[mt][r] = frac32(A.f_s * tau / A.N);
+            if (TAN) Ttx[TAN ? mt : 0][r] = (float)tg;
         }
     const int k0 = kelem<PREC>(0, g);
     const float dg = (float)(((double)k0 - A.M) * A.pitch - x);  // x_k - x of element j = 0
@@ -366,6 +390,10 @@ void beamform_kernel(BeamArgs A)
     const float dg_tx = (float)((4.0 * g - A.M) * A.pitch - x);  // SA transmit rows, mt = r = 0
 
     float Ir = 0.f, Ii = 0.f, w_last = 0.f, wtx_last = 0.f;
+    float Dr = 0.f, Di = 0.f;  // tangent accumulators
+    // w_n = n * 2 pi f_s / N; the tangent factor -w_n / c_tof is applied per bin
+    const float w_unit = (float)(2.0 * 3.141592653589793 * A.f_s / A.N);
+    const float inv_c = (float)(1.0 / A.ctof);
     const int b_begin = blockIdx.y * A.bin_chunk;
     const int b_end = min(b_begin + A.bin_chunk, A.n_f);
     const int MT_all = (A.n_tx + 15) >> 4;  // stride of the operand layout
@@ -403,6 +431,7 @@ void beamform_kernel(BeamArgs A)
         const Aperture ap = make_aperture(x, xp, zp, Fr, A, A.support_rx);
 
         floatx4 accR[MT], accI[MT];
+        floatx4 tanR[TAN ? MT : 1], tanI[TAN ? MT : 1];
         float asum = 0.f;
         // recompute x_k - x per bin (one FMA each) instead of holding KI registers
         float dgb = dg;
@@ -413,11 +442,17 @@ void beamform_kernel(BeamArgs A)
             //   Re Y = K1 - K3, Im Y = K1 + K2
             const float2 *xs = reinterpret_cast<const float2 *>(smem_raw);
             floatx4 K1[MT], K2[MT], K3[MT];
+            floatx4 T1[TAN ? MT : 1], T2[TAN ? MT : 1], T3[TAN ? MT : 1];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 K1[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
                 K2[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
                 K3[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                if (TAN) {
+                    T1[TAN ? mt : 0] = floatx4{0.f, 0.f, 0.f, 0.f};
+                    T2[TAN ? mt : 0] = floatx4{0.f, 0.f, 0.f, 0.f};
+                    T3[TAN ? mt : 0] = floatx4{0.f, 0.f, 0.f, 0.f};
+                }
             }
             const int klo = ap.klo - g, khi = ap.khi - g;  // support in units of 4j
             const float dstep = 4.0f * pf;
@@ -438,12 +473,23 @@ void beamform_kernel(BeamArgs A)
                     K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xsum, br, K1[mt], 0, 0, 0);
                     K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, bd, K2[mt], 0, 0, 0);
                     K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, bp, K3[mt], 0, 0, 0);
+                    if (TAN) {
+                        const float tr = Trx[TAN ? j : 0];
+                        T1[TAN ? mt : 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xsum, br * tr, T1[TAN ? mt : 0], 0, 0, 0);
+                        T2[TAN ? mt : 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, bd * tr, T2[TAN ? mt : 0], 0, 0, 0);
+                        T3[TAN ? mt : 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, bp * tr, T3[TAN ? mt : 0], 0, 0, 0);
+                    }
                 }
             }
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 accR[mt] = K1[mt] - K3[mt];
                 accI[mt] = K1[mt] + K2[mt];
+                if (TAN) {
+                    // Y' = sum_k B tau_rx,g X, then Y' += tau_tx,g Y (row factor)
+                    tanR[TAN ? mt : 0] = T1[TAN ? mt : 0] - T3[TAN ? mt : 0];
+                    tanI[TAN ? mt : 0] = T1[TAN ? mt : 0] + T2[TAN ? mt : 0];
+                }
             }
         } else {
             // 16-bit operands, FP32 accumulation: Re Y = Xr Br - Xi Bi, Im Y = Xr Bi + Xi Br;
@@ -510,6 +556,7 @@ void beamform_kernel(BeamArgs A)
             if (ap.valid) {
                 const float sc = (A.norm_rx ? 1.0f / asum : 1.0f) * xscale;
                 // padded transmit rows have zero spectra, so every row is summed
+                float pr = 0.f, pi = 0.f;  // tangent: sum_rows T (tau_tx,g Y + Y')
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -519,7 +566,20 @@ void beamform_kernel(BeamArgs A)
                         const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
                         Ir += cs * yr - sn * yi;
                         Ii += cs * yi + sn * yr;
+                        if (TAN) {
+                            const float tt = Ttx[TAN ? mt : 0][r];
+                            const float ur = fmaf(tt, accR[mt][r], tanR[TAN ? mt : 0][r]);
+                            const float ui = fmaf(tt, accI[mt][r], tanI[TAN ? mt : 0][r]);
+                            pr += cs * ur - sn * ui;
+                            pi += cs * ui + sn * ur;
+                        }
                     }
+                if (TAN) {
+                    // dI += sc (j w_n) (-1 / c) P
+                    const float f = -sc * (float)n * w_unit * inv_c;
+                    Dr = fmaf(-f, pi, Dr);
+                    Di = fmaf(f, pr, Di);
+                }
             }
         } else {
             // transmit aperture and apodization (das_sa.m:384-400, 438-455)
@@ -565,13 +625,25 @@ void beamform_kernel(BeamArgs A)
     Ir += __shfl_xor(Ir, 32);
     Ii += __shfl_xor(Ii, 16);
     Ii += __shfl_xor(Ii, 32);
+    if (TAN) {
+        Dr += __shfl_xor(Dr, 16);
+        Dr += __shfl_xor(Dr, 32);
+        Di += __shfl_xor(Di, 16);
+        Di += __shfl_xor(Di, 32);
+    }
     if (g == 0 && vok) {
         const float2 val = make_float2(A.out_scale * Ir, A.out_scale * Ii);
+        float2 *dimg = reinterpret_cast<float2 *>(A.dimage);
         if (A.atomic_out) {
             atomicAdd(&A.image[v].x, val.x);
             atomicAdd(&A.image[v].y, val.y);
+            if (TAN) {
+                atomicAdd(&dimg[v].x, A.out_scale * Dr);
+                atomicAdd(&dimg[v].y, A.out_scale * Di);
+            }
         } else {
             A.image[v] = val;
+            if (TAN) dimg[v] = make_float2(A.out_scale * Dr, A.out_scale * Di);
         }
         if (b_end == A.n_f && blockIdx.z == 0) {
             if (A.weights) A.weights[v] = w_last;

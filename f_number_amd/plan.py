@@ -88,8 +88,34 @@ class _Plan:
         self.close()
 
 
+def _rf_dtype(rf, prefix):
+    import torch
+
+    if rf.dtype == torch.float32:
+        return _lib.DTYPE["float32"]
+    if rf.dtype == torch.float64:
+        return _lib.DTYPE["float64"]
+    raise _lib.FNumberError(f"{prefix}:InvalidRFData", "RF must be float32 or float64")
+
+
+def _stream(stream, device):
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream(device)
+    return C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+def _need(t, dtype, numel, what, prefix):
+    if t is None or not t.is_cuda or not t.is_contiguous() or t.dtype != dtype or t.numel() != numel:
+        raise _lib.FNumberError(f"{prefix}:InvalidArgument",
+                                f"{what} must be a contiguous {dtype} device tensor of {numel} elements")
+
+
 class PwPlan(_Plan):
-    """Plan of das_pw for a fixed grid, acquisition and angle set."""
+    """Plan of das_pw for a fixed grid, acquisition and angle set.
+
+    Differentiation form: set_tof, execute_tangent, adjoint and vjp (fnumber_hip.h)."""
     prefix = "gpu_bf_das_pw_rf"
 
     def __init__(self, positions_x, positions_z, n_samples, n_elements, steering_angles, f_s, element_width,
@@ -105,7 +131,66 @@ class PwPlan(_Plan):
         _lib.check(_lib.load().fn_plan_pw_create(C.byref(P), C.byref(self._h)), self.prefix)
         self._keep = keep
         self.grid = info
-        self._rf_numel = int(n_samples) * int(n_elements) * len(keep[2])
+        self.n_samples, self.n_elements, self.n_angles = int(n_samples), int(n_elements), len(keep[2])
+        self._rf_numel = self.n_samples * self.n_elements * self.n_angles
+
+    def set_tof(self, c_tof=0.0, delay_offsets=None):
+        """Speed of sound of the time of flight (0: the plan's c_0) and per-element
+        receive delay offsets in seconds (None: zero); apodization stays at c_0."""
+        import numpy as np
+
+        d = None
+        if delay_offsets is not None:
+            d = np.ascontiguousarray(delay_offsets, dtype=np.float64)
+            if d.size != self.n_elements:
+                raise _lib.FNumberError(f"{self.prefix}:InvalidArgument", "one delay offset per element")
+        ptr = d.ctypes.data_as(C.POINTER(C.c_double)) if d is not None else None
+        _lib.check(_lib.load().fn_plan_set_tof(self._h, float(c_tof), ptr), self.prefix)
+
+    def execute_tangent(self, rf, image, dimage, weights=None, stream=None):
+        """image and dI/dc_tof (complex64 (N_x, N_z) each)."""
+        import torch
+
+        n = self.info()["n_voxels"]
+        _need(image, torch.complex64, n, "image", self.prefix)
+        _need(dimage, torch.complex64, n, "dimage", self.prefix)
+        if rf.numel() != self._rf_numel or not rf.is_cuda or not rf.is_contiguous():
+            raise _lib.FNumberError(f"{self.prefix}:InvalidRFData", "RF size does not match the plan")
+        if weights is not None:
+            _need(weights, torch.float32, n, "weights", self.prefix)
+        _lib.check(_lib.load().fn_plan_execute_tangent(self._h, _ptr(rf), _rf_dtype(rf, self.prefix), _ptr(image),
+                                                       _ptr(dimage), _ptr(weights), _stream(stream, rf.device)),
+                   self.prefix)
+
+    def adjoint(self, y, rf_adj=None, stream=None):
+        """A^H y: y complex64 (N_x, N_z) -> float32 (N_a, N_el, N_t)."""
+        import torch
+
+        _need(y, torch.complex64, self.info()["n_voxels"], "y", self.prefix)
+        if rf_adj is None:
+            rf_adj = torch.empty((self.n_angles, self.n_elements, self.n_samples), dtype=torch.float32,
+                                 device=y.device)
+        _need(rf_adj, torch.float32, self._rf_numel, "rf_adj", self.prefix)
+        _lib.check(_lib.load().fn_plan_adjoint(self._h, _ptr(y), _ptr(rf_adj), _stream(stream, y.device)),
+                   self.prefix)
+        return rf_adj
+
+    def vjp(self, rf, g, dimage=None, stream=None):
+        """(dJ/d delay [N_el] float64, dJ/dc_tof [1] float64 or None) for cotangent g
+        (dJ = sum Re(conj(g) dI)); dJ/dc needs dimage from execute_tangent."""
+        import torch
+
+        n = self.info()["n_voxels"]
+        _need(g, torch.complex64, n, "g", self.prefix)
+        if dimage is not None:
+            _need(dimage, torch.complex64, n, "dimage", self.prefix)
+        if rf.numel() != self._rf_numel or not rf.is_cuda or not rf.is_contiguous():
+            raise _lib.FNumberError(f"{self.prefix}:InvalidRFData", "RF size does not match the plan")
+        vd = torch.empty(self.n_elements, dtype=torch.float64, device=g.device)
+        vc = torch.empty(1, dtype=torch.float64, device=g.device) if dimage is not None else None
+        _lib.check(_lib.load().fn_plan_vjp(self._h, _ptr(rf), _rf_dtype(rf, self.prefix), _ptr(g), _ptr(dimage),
+                                           _ptr(vd), _ptr(vc), _stream(stream, g.device)), self.prefix)
+        return vd, vc
 
 
 class SaPlan(_Plan):

@@ -199,8 +199,9 @@ int32_t fn_das_sa(const fn_sa_params *p, const void *data_rf, int32_t rf_dtype, 
 /* ---- plans: device-resident execution ---------------------------------- */
 /* A plan holds the validated configuration, the device copies of the grid and
  * F-number tables and the spectra workspace for one shape.  fn_plan_execute
- * takes device pointers and enqueues on `stream` (a hipStream_t, NULL = the
- * plan's own stream) without host synchronisation. */
+ * takes device pointers and enqueues on `stream` (a hipStream_t; NULL is the
+ * default stream, as in HIP) without host synchronisation.  Every fn_plan_*
+ * call that takes a stream orders its work on that stream only. */
 typedef struct fn_plan fn_plan;
 typedef struct fn_plan_info {
     int32_t N_dft, n_lb, n_ub, n_f;
@@ -222,6 +223,33 @@ int32_t fn_plan_execute(fn_plan *plan, const void *d_rf, int32_t rf_dtype, float
 int32_t fn_plan_set_profiling(fn_plan *plan, int32_t enable);
 int32_t fn_plan_get_profile(fn_plan *plan, double *spectra_ms, double *beamform_ms, int32_t *n_runs);
 int32_t fn_plan_destroy(fn_plan *plan);
+
+/* ---- differentiation form (plane-wave plans) ---------------------------- */
+/* No counterpart in the reference (its MEX only beamforms); these extend the
+ * plan of fn_plan_pw_create with the time-of-flight parameters
+ *   theta = (c_tof, delay_k): tau_tx = geometric_tx / c_tof + i_t0 / f_s,
+ *                              tau_rx(k) = geometric_rx(k) / c_tof + delay_k
+ * (das_pw.m:336, 339-340, 427-428 with c_0 -> c_tof and the offsets added).
+ * Frequency grid, F-numbers, apertures and apodization stay those of the
+ * plan's c_0, so the image is smooth in theta.
+ * c_tof <= 0 restores c_0; delay_offsets (host, [n_elements], seconds) NULL = 0.
+ * Also applies to fn_plan_execute (SA plans: the offset of element a delays
+ * both its transmit and its receive). */
+int32_t fn_plan_set_tof(fn_plan *plan, double c_tof, const double *delay_offsets);
+/* image and its tangent dI/dc_tof (same layout), FP32; d_weights may be NULL */
+int32_t fn_plan_execute_tangent(fn_plan *plan, const void *d_rf, int32_t rf_dtype, float *d_image,
+                                float *d_dimage_dc, float *d_weights, void *stream);
+/* adjoint (backprojection) of the RF -> image map of fn_plan_execute:
+ * d_y complex float [n_x][n_z] -> d_rf_adj float [n_tx][n_el][n_t] with
+ * Re sum conj(A x) y = sum x (A^H y) (same output scale as the forward). */
+int32_t fn_plan_adjoint(fn_plan *plan, const float *d_y, float *d_rf_adj, void *stream);
+/* vector-Jacobian products for a cotangent d_g (complex float [n_x][n_z]):
+ * d_vjp_delay[k] = sum_v Re(conj(g_v) dI_v / d delay_k)   (double [n_el], device)
+ * and, when d_dimage_dc (from fn_plan_execute_tangent) and d_vjp_c are given,
+ * d_vjp_c[0] = sum_v Re(conj(g_v) dI_v / dc_tof)           (double, device).
+ * Recomputes the spectra of d_rf. */
+int32_t fn_plan_vjp(fn_plan *plan, const void *d_rf, int32_t rf_dtype, const float *d_g,
+                    const float *d_dimage_dc, double *d_vjp_delay, double *d_vjp_c, void *stream);
 
 /* ---- synthetic acquisitions (seeded; the reference ships no data) ------ */
 typedef struct fn_synth_params {

@@ -465,7 +465,7 @@ static double f_at(double F_n, double f_n, double z, double c_0, int clamp)
 /* ------------------------------------------------------------------------- */
 /* das_pw                                                                    */
 /* ------------------------------------------------------------------------- */
-static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offsets)
+static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offsets, or_pw_diff_t *dx)
 {
     const int n_el = a->n_el;
     const double M = (n_el - 1) / 2.0;
@@ -497,6 +497,20 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
     const int clamp = (a->semantics == OR_SEM_CPU);
     const int nvox = a->n_sel > 0 ? a->n_sel : a->n_x * a->n_z;
     for (long v = 0; v < nvox; ++v) { a->image_re[v] = 0.0; a->image_im[v] = 0.0; }
+    const int want_dc = dx && dx->dIdc_re;
+    const int want_vjp = dx && dx->g_re;
+    const int want_adj = dx && dx->y_re;
+    const long nz_adj = want_adj ? (long)a->n_tx * n_el * nf : 0;
+    double *Zr_all = NULL, *Zi_all = NULL;
+    double vjp_c = 0.0;
+    if (want_dc)
+        for (long v = 0; v < nvox; ++v) { dx->dIdc_re[v] = 0.0; dx->dIdc_im[v] = 0.0; }
+    if (want_vjp)
+        for (int k = 0; k < n_el; ++k) dx->vjp_delay[k] = 0.0;
+    if (want_adj) {
+        Zr_all = (double *)calloc(nz_adj, sizeof(double));
+        Zi_all = (double *)calloc(nz_adj, sizeof(double));
+    }
 
 #ifdef _OPENMP
     if (a->n_threads > 0) omp_set_num_threads(a->n_threads);
@@ -504,6 +518,11 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
 #pragma omp parallel
     {
         double *tof = (double *)malloc(sizeof(double) * n_el);
+        double *dtof = (double *)malloc(sizeof(double) * n_el);  /* d tof / d c_tof */
+        double *vjp = want_vjp ? (double *)calloc(n_el, sizeof(double)) : NULL;
+        double *Zr = want_adj ? (double *)calloc(nz_adj, sizeof(double)) : NULL;
+        double *Zi = want_adj ? (double *)calloc(nz_adj, sizeof(double)) : NULL;
+        double vjp_c_t = 0.0;
         double *w = (double *)malloc(sizeof(double) * (long)nf * n_el);
         int *vidx = (int *)malloc(sizeof(int) * nf);
         aperture_t *aps = (aperture_t *)malloc(sizeof(aperture_t) * nf);
@@ -536,19 +555,25 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
             }
             if (a->norm_rx == OR_NORM_WINDOWED) windowed_taper(w, nfv, n_el, a->norm_rx_win, a->norm_rx_win_param);
 
-            double acc_r = 0.0, acc_i = 0.0;
+            double acc_r = 0.0, acc_i = 0.0, dc_r = 0.0, dc_i = 0.0;
+            const double gr = want_vjp ? dx->g_re[v] : 0.0, gi = want_vjp ? dx->g_im[v] : 0.0;
+            const double yr = want_adj ? dx->y_re[v] : 0.0, yi = want_adj ? dx->y_im[v] : 0.0;
             for (int ia = 0; ia < a->n_tx; ++ia) {
                 const double th = a->angles[ia];
                 const double ex = -sin(th), ez = cos(th);
                 const double sg = (ex > 0) - (ex < 0);
                 const double xref = sg * ((0 - M) * a->pitch);
                 /* das_pw.m:336, 339-340, 427-428 */
-                const double t_in = (ex * (x - xref) + ez * z) / c_tof + a->index_t0 / a->f_s;
+                const double t_geo = (ex * (x - xref) + ez * z) / c_tof;
+                const double t_in = t_geo + a->index_t0 / a->f_s;
                 const double zc = z / c_tof;
                 for (int k = 0; k < n_el; ++k) {
                     const double lat = (((k - M) * a->pitch) - x) / c_tof;
-                    tof[k] = t_in + sqrt(lat * lat + zc * zc);
+                    const double t_rx = sqrt(lat * lat + zc * zc);
+                    tof[k] = t_in + t_rx;
                     if (delay_offsets) tof[k] += delay_offsets[k];
+                    /* geometric times scale as 1 / c: d/dc = -(t_geo + t_rx) / c */
+                    dtof[k] = -(t_geo + t_rx) / c_tof;
                 }
                 double img_r = 0.0, img_i = 0.0;
                 for (int j = 0; j < nfv; ++j) {
@@ -562,8 +587,21 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
                         const double ww = w[(long)j * n_el + k];
                         /* (window .* weights) .* data */
                         const double pr = ww * wr, pi_ = ww * wi;
-                        fr += pr * xr - pi_ * xi;
-                        fi += pr * xi + pi_ * xr;
+                        const double tr = pr * xr - pi_ * xi, ti = pr * xi + pi_ * xr;
+                        fr += tr;
+                        fi += ti;
+                        if (want_dc || want_vjp) {
+                            /* d/dtau of the term: j om term */
+                            const double dr = -om[i] * ti, di = om[i] * tr;
+                            if (want_dc) { dc_r += dtof[k] * dr; dc_i += dtof[k] * di; }
+                            if (want_vjp) vjp[k] += gr * dr + gi * di; /* Re(conj(g) dI/d delay_k) */
+                        }
+                        if (want_adj) {
+                            /* adjoint: y conj(2 ww e^{j arg}) accumulated per (angle, element, bin) */
+                            const long zi = ((long)ia * n_el + k) * nf + i;
+                            Zr[zi] += 2 * ww * (yr * wr + yi * wi);
+                            Zi[zi] += 2 * ww * (yi * wr - yr * wi);
+                        }
                     }
                     img_r += fr;
                     img_i += fi;
@@ -573,18 +611,50 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
             }
             a->image_re[v] = acc_r;
             a->image_im[v] = acc_i;
+            if (want_dc) { dx->dIdc_re[v] = dc_r; dx->dIdc_im[v] = dc_i; }
+            if (want_vjp && want_dc) vjp_c_t += gr * dc_r + gi * dc_i;
         }
-        free(tof); free(w); free(vidx); free(aps);
+#pragma omp critical
+        {
+            if (want_vjp) for (int k = 0; k < n_el; ++k) dx->vjp_delay[k] += vjp[k];
+            vjp_c += vjp_c_t;
+            if (want_adj) for (long q = 0; q < nz_adj; ++q) { Zr_all[q] += Zr[q]; Zi_all[q] += Zi[q]; }
+        }
+        free(tof); free(dtof); free(w); free(vidx); free(aps); free(vjp); free(Zr); free(Zi);
+    }
+    if (want_vjp) dx->vjp_c = vjp_c;
+    if (want_adj) {
+        /* band-limited inverse DFT of Z back to the RF samples t < n_t:
+           out[ia][k][t] = Re sum_n Z[ia][k][n] e^{+j 2 pi n t / N}, index n t reduced mod N exactly */
+        const long nch2 = (long)a->n_tx * n_el;
+#pragma omp parallel for schedule(static)
+        for (long ch = 0; ch < nch2; ++ch)
+            for (int t = 0; t < a->n_t; ++t) {
+                double acc = 0.0;
+                for (int i = 0; i < nf; ++i) {
+                    const long m = ((long)(nlb + i) * t) % N;
+                    const double ang = 2 * M_PI * (double)m / N;
+                    acc += Zr_all[ch * nf + i] * cos(ang) - Zi_all[ch * nf + i] * sin(ang);
+                }
+                dx->adj_out[ch * a->n_t + t] = acc;
+            }
+        free(Zr_all);
+        free(Zi_all);
     }
     free(fax); free(om); free(r); free(Fv); free(Xr); free(Xi);
     return 0;
 }
 
-int or_das_pw(or_das_args_t *a) { return das_pw_impl(a, a->c_0, NULL); }
+int or_das_pw(or_das_args_t *a) { return das_pw_impl(a, a->c_0, NULL, NULL); }
 
 int or_das_pw_c(or_das_args_t *a, double c_tof, const double *delay_offsets)
 {
-    return das_pw_impl(a, c_tof, delay_offsets);
+    return das_pw_impl(a, c_tof, delay_offsets, NULL);
+}
+
+int or_das_pw_diff(or_das_args_t *a, double c_tof, const double *delay_offsets, or_pw_diff_t *d)
+{
+    return das_pw_impl(a, c_tof, delay_offsets, d);
 }
 
 /* ------------------------------------------------------------------------- */

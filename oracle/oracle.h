@@ -99,6 +99,25 @@ int or_das_sa(or_das_args_t *a);
    the differentiation form).  Same as or_das_pw but TOF uses c_tof. */
 int or_das_pw_c(or_das_args_t *a, double c_tof, const double *delay_offsets);
 
+/* Derivatives of the PW image with respect to the time-of-flight parameters
+ * (speed of sound c_tof of the geometric delays, per-element receive delay
+ * offsets), apertures and apodization frozen at c_0; and the adjoint of the
+ * RF -> image map.  Each output is optional (NULL pointer = not computed):
+ *   dIdc            image-shaped dI/dc_tof
+ *   g -> vjp_delay  vjp_delay[k] = sum_v Re(conj(g_v) dI_v / d delay_k)
+ *   g + dIdc -> vjp_c = sum_v Re(conj(g_v) dI_v / dc_tof)
+ *   y -> adj_out    A^H y, [n_tx][n_el][n_t], with <A x, y> = Re sum conj(A x) y = <x, A^H y>
+ * Scale: the image of or_das_pw (2 * sum). */
+typedef struct {
+    double *dIdc_re, *dIdc_im;
+    const double *g_re, *g_im;
+    double *vjp_delay;
+    double vjp_c;
+    const double *y_re, *y_im;
+    double *adj_out;
+} or_pw_diff_t;
+int or_das_pw_diff(or_das_args_t *a, double c_tof, const double *delay_offsets, or_pw_diff_t *d);
+
 #ifdef __cplusplus
 }
 #endif

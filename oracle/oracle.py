@@ -88,6 +88,7 @@ def lib():
         L.or_das_pw.argtypes = [C.POINTER(_Args)]
         L.or_das_sa.argtypes = [C.POINTER(_Args)]
         L.or_das_pw_c.argtypes = [C.POINTER(_Args), C.c_double, d]
+        L.or_das_pw_diff.argtypes = [C.POINTER(_Args), C.c_double, d, C.POINTER(_Diff)]
         _lib = L
     return _lib
 
@@ -210,6 +211,80 @@ def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
     if sel is None:
         img = img.reshape(np.asarray(x).size, np.asarray(z).size).T
     return img, dict(N_dft=a.N_dft, n_lb=a.n_lb, n_ub=a.n_ub)
+
+
+class _Diff(C.Structure):
+    _fields_ = [("dIdc_re", C.POINTER(C.c_double)), ("dIdc_im", C.POINTER(C.c_double)),
+                ("g_re", C.POINTER(C.c_double)), ("g_im", C.POINTER(C.c_double)),
+                ("vjp_delay", C.POINTER(C.c_double)), ("vjp_c", C.c_double),
+                ("y_re", C.POINTER(C.c_double)), ("y_im", C.POINTER(C.c_double)),
+                ("adj_out", C.POINTER(C.c_double))]
+
+
+def das_pw_diff(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
+                window=("tukey", 0.2), F=None, normalization="on", semantics="cpu",
+                spectra_mode=1, n_threads=0, norm_window=("boxcar", 0.0),
+                c_tof=None, delay_offsets=None, g=None, y=None, want_dc=False):
+    """FP64 derivatives of the PW image (oracle.h or_das_pw_diff).
+
+    g, y: complex [n_z, n_x] (image layout).  Returns a dict with 'image',
+    'dIdc' (want_dc), 'vjp_delay' and 'vjp_c' (g given; vjp_c also needs
+    want_dc), 'adjoint' [n_tx, n_el, n_t] (y given)."""
+    rf = np.asarray(rf)
+    if rf.ndim == 2:
+        rf = rf[None]
+    n_tx, n_el, n_t = rf.shape
+    angles = np.ascontiguousarray(np.atleast_1d(angles), dtype=np.float64)
+    a, keep, re, im = _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics,
+                            spectra_mode, n_threads, None)
+    a.n_t, a.n_el, a.n_tx = n_t, n_el, n_tx
+    a.angles = _dp(angles)
+    a.win_rx, a.win_rx_param = WIN[window[0]], float(window[1])
+    a.f_rx = F if F is not None else fnumber("angle_lb", 60, 1.5, 10)
+    a.norm_rx = NORM[normalization]
+    a.norm_rx_win, a.norm_rx_win_param = WIN[norm_window[0]], float(norm_window[1])
+    nx, nz = np.asarray(x).size, np.asarray(z).size
+    nv = nx * nz
+    d = _Diff()
+    out = {}
+
+    def vox(img):  # [n_z, n_x] -> oracle order (x-major, z fastest)
+        img = np.asarray(img, dtype=np.complex128).T.ravel()
+        r, i = np.ascontiguousarray(img.real), np.ascontiguousarray(img.imag)
+        keep.extend([r, i])
+        return r, i
+
+    if want_dc:
+        dr, di = np.zeros(nv), np.zeros(nv)
+        d.dIdc_re, d.dIdc_im = _dp(dr), _dp(di)
+    if g is not None:
+        gr, gi = vox(g)
+        d.g_re, d.g_im = _dp(gr), _dp(gi)
+        vj = np.zeros(n_el)
+        d.vjp_delay = _dp(vj)
+    if y is not None:
+        yr, yi = vox(y)
+        d.y_re, d.y_im = _dp(yr), _dp(yi)
+        adj = np.zeros(n_tx * n_el * n_t)
+        d.adj_out = _dp(adj)
+    offs = None
+    if delay_offsets is not None:
+        offs = np.ascontiguousarray(delay_offsets, dtype=np.float64)
+    rc = lib().or_das_pw_diff(C.byref(a), float(c_0 if c_tof is None else c_tof),
+                              _dp(offs) if offs is not None else None, C.byref(d))
+    if rc:
+        raise ValueError(f"or_das_pw_diff failed ({rc})")
+    out["image"] = (re + 1j * im).reshape(nx, nz).T
+    if want_dc:
+        out["dIdc"] = (dr + 1j * di).reshape(nx, nz).T
+    if g is not None:
+        out["vjp_delay"] = vj
+        if want_dc:
+            out["vjp_c"] = d.vjp_c
+    if y is not None:
+        out["adjoint"] = adj.reshape(n_tx, n_el, n_t)
+    out["info"] = dict(N_dft=a.N_dft, n_lb=a.n_lb, n_ub=a.n_ub)
+    return out
 
 
 def das_sa(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0=0.0,

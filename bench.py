@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--workload", choices=["cpwc_full", "cpwc_small", "saft_full"], default="cpwc_full")
     ap.add_argument("--precisions", default="bf16x3,fp16x3,fp16",
                     help="mixed-precision arms of the headline configuration (comma list, '' for none)")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form measurements")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
@@ -109,6 +110,67 @@ def run_arm(w, plan_kw, steps, warmup, dev, keep_image=False):
         "beamform_ms": beam_ms / max(n, 1),
         "info": info,
         "finite": ok,
+    }
+
+
+def _time_ms(fn, steps, warmup, dev):
+    """mean ms per call of fn() from events recorded on the current stream"""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize(dev)
+    return e0.elapsed_time(e1) / steps
+
+
+def run_diff(steps, warmup, dev):
+    """Differentiation form on the layered-medium acquisition (11 angles, 2048 samples,
+    384 x 384): forward, adjoint, tangent and vector-Jacobian timings (all enqueued on
+    torch's current stream; NULL stream handle = the default stream) and a 50-iteration
+    autofocus from c = 1480 m/s."""
+    import numpy as np
+    import torch
+
+    import f_number_amd as fa
+    from f_number_amd import autofocus as AF
+    from f_number_amd import synth
+
+    w = synth.autofocus_layer()
+    rf = w.rf_device(dev)
+    kw = dict(window=fa.windows.tukey(0.5), F_number=fa.f_numbers.grating.angle_lb(60, 1.5),
+              normalization=fa.normalizations.on())
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     1480.0, w.f_bounds, 0.0, **kw)
+    n = (w.positions_x.size, w.positions_z.size)
+    img = torch.empty(n, dtype=torch.complex64, device=dev)
+    dimg = torch.empty_like(img)
+    adj = torch.empty((w.n_tx, w.n_elements, w.n_samples), dtype=torch.float32, device=dev)
+    plan.set_tof(1500.0, w.delay_offsets * 0.5)
+    t_fwd = _time_ms(lambda: plan.execute(rf, img), steps, warmup, dev)
+    y = img.clone()
+    t_adj = _time_ms(lambda: plan.adjoint(y, adj), steps, warmup, dev)
+    t_tan = _time_ms(lambda: plan.execute_tangent(rf, img, dimg), steps, warmup, dev)
+    g = (2 * img).contiguous()
+    t_vjp = _time_ms(lambda: plan.vjp(rf, g, dimg), steps, warmup, dev)
+    res = AF.autofocus(plan, rf, n_iter=50, c0=1480.0)
+    plan.close()
+    truth = w.delay_offsets - w.delay_offsets.mean()
+    d = res["delays"] - res["delays"].mean()
+    h = res["history"]
+    return {
+        "workload": w.name, "n_voxels": int(w.n_voxels),
+        "forward_ms": t_fwd, "adjoint_ms": t_adj, "tangent_ms": t_tan, "vjp_ms": t_vjp,
+        "forward_plus_adjoint_throughput_vs_forward": t_fwd / (t_fwd + t_adj),
+        "autofocus": {"iterations": 50, "seconds": res["seconds"], "c_start": 1480.0, "c_final": res["c"],
+                      "delay_rms_error_ns_start": float(np.std(truth) * 1e9),
+                      "delay_rms_error_ns_final": float(np.std(d - truth) * 1e9),
+                      "objective_gain": h[-1]["J"] / h[0]["J"]},
     }
 
 
@@ -197,6 +259,8 @@ def main():
                  for k, v in arms.items()},
         "precision_arms": prec_arms,
     }
+    if not args.no_diff:
+        out["differentiation"] = run_diff(max(3, args.steps // 2), 1, dev)
     if not args.no_cpu_baseline:
         nthreads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(w, "hann", "angle_lb", (60, 1.5, 10), nthreads)

@@ -87,6 +87,9 @@ class SynthParams(C.Structure):
         ("f_c", C.c_double), ("fractional_bandwidth", C.c_double), ("snr_db", C.c_double),
         ("mode", C.c_int32), ("n_angles", C.c_int32), ("steering_angles", _dp),
         ("on_device", C.c_int32), ("index_gpu", C.c_int32),
+        ("layer_thickness", C.c_double), ("layer_c", C.c_double), ("delay_offsets", _dp),
+        ("tube_x", C.c_double), ("tube_z", C.c_double), ("tube_radius", C.c_double), ("tube_angle", C.c_double),
+        ("tube_v_max", C.c_double), ("frame_time", C.c_double),
     ]
 
 
@@ -120,6 +123,7 @@ PROTOTYPES = {
     "fn_plan_adjoint": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fn_plan_vjp": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
+    "fn_das_pw_adjoint": (C.c_int32, [C.POINTER(PwParams), C.c_double, _dp, _fp, _fp]),
     "fn_synth_rf": (C.c_int32, [C.POINTER(SynthParams), C.c_void_p]),
     "fn_synth_scatterers": (C.c_int32, [C.POINTER(SynthParams), _dp, _dp, _dp]),
 }

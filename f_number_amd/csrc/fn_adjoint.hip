@@ -14,12 +14,15 @@
 // Voxel contraction on the matrix cores (v_mfma_f32_16x16x4f32, Gauss three
 // multiplications): per bin, D[element][angle] += sum_v conj(B)[element][v] U[v][angle]
 // with U = s y conj(T) / w.  The contraction index (voxels) is the MFMA K, so
-// lane l holds the pairs (voxel 4s + l/16 of its 16-voxel group, element
-// l%16 + 16 mt); apertures are computed per voxel in the forward layout and
-// exchanged with lane shuffles.  A block owns a chunk of 8 bins and a voxel
-// range; it sums the wave results of each bin in LDS and writes one partial
-// per (range, bin), reduced by a second kernel.  Phase tiles are recomputed
-// per bin, never stored.
+// lane l holds the pairs (voxel 4s + l/16 of a 16-voxel group, element
+// l%16 + 16 mt).  A block of 8 waves owns 8 consecutive bins (one per wave)
+// and a voxel range: per 16-voxel group the block computes the phase words
+// (FP64 time of flight reduced to 2^-32 revolutions) once into LDS, every wave
+// contracts the group for its own bin and keeps its 16 x 128 x 16 result in
+// registers across the whole range -- no atomics; one partial per (range, bin)
+// is written at the end and reduced by a second kernel.  Apertures are
+// computed per voxel in the forward layout and exchanged with lane shuffles.
+// Phase tiles are recomputed per bin, never stored.
 //
 // Vector-Jacobian products for a cotangent g (J real, dJ = sum Re(conj(g) dI)):
 //   d J / d delay_k = sum_n w_n sum_a Im(conj(X_n(k,a)) Z_n(k,a))   with Z from y = g
@@ -31,156 +34,144 @@
 
 namespace fnk {
 
-constexpr int ADJ_C = 8;       // bins per block
-constexpr int ADJ_WAVES = 4;   // 256 threads, one wave per SIMD (the accumulators need the full register file)
+constexpr int ADJ_WAVES = 8;   // bins per block, one per wave
 constexpr int ADJ_ET = 8;      // element tiles of 16 (128 elements)
-constexpr int ADJ_PAD = 17;    // LDS row stride (floats) of one 16 x 16 tile
-constexpr int ADJ_TILE = 16 * ADJ_PAD;
-constexpr size_t ADJ_LDS = (size_t)ADJ_C * 2 * ADJ_ET * ADJ_TILE * sizeof(float);
+constexpr int ADJ_QS = 132;    // LDS words per voxel of the receive phase table ([cidx][mt] + pad)
 
 template <int WCR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void adjoint_kernel(AdjArgs S)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(AdjArgs S)
 {
-    extern __shared__ __align__(16) float zacc[];  // [ADJ_C][2][ADJ_ET][16][ADJ_PAD]
+    __shared__ __align__(16) uint32_t q_rx[16 * ADJ_QS];  // [voxel i][cidx][mt]: element cidx + 16 mt
+    __shared__ uint32_t q_tx[16 * 16];      // [voxel i][angle column]
+    __shared__ double vxz[2 * 16];          // x, z of the group's voxels
+    __shared__ float2 vy[16];               // cotangent of the group's voxels (0 outside the range)
     const BeamArgs &A = S.B;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cidx = lane & 15, q = lane >> 4;
     const long n_vox = (long)A.n_x * A.n_z;
-    const int b0 = blockIdx.y * ADJ_C;
-    const int nb = min(ADJ_C, A.n_f - b0);
+    const int b = blockIdx.y * ADJ_WAVES + wave;
+    const bool active = b < A.n_f;
     const int at = blockIdx.z;
     const long vr0 = (long)blockIdx.x * S.vox_per_range;
     const long vr1 = min(vr0 + S.vox_per_range, n_vox);
-
-    for (int i = threadIdx.x; i < ADJ_C * 2 * ADJ_ET * ADJ_TILE; i += 64 * ADJ_WAVES) zacc[i] = 0.f;
-    __syncthreads();
-
     const float pf = (float)A.pitch;
-    const int a_lane = 16 * at + cidx;  // angle of this lane's U column
-    double ex = 0.0, ez = 0.0, xr = 0.0;
-    if (a_lane < A.n_tx) {
-        ex = A.txp[3 * a_lane];
-        ez = A.txp[3 * a_lane + 1];
-        xr = A.txp[3 * a_lane + 2];
+    const uint32_t n = (uint32_t)(A.n_lb + (active ? b : 0));
+    double Fb = 0.0, sqf = 0.0;
+    if (active) {
+        Fb = A.Frx[b];
+        sqf = A.sqrt_f[b];
     }
 
-    for (long base = vr0 + 16 * wave; base < vr1; base += 16 * ADJ_WAVES) {
-        // forward layout: voxel base + cidx computes the apertures
-        const long vf = min(base + cidx, n_vox - 1);
-        const int ixf = (int)(vf / A.n_z), izf = (int)(vf - (long)ixf * A.n_z);
-        const double xf = A.px[ixf], zf = A.pz[izf];
-        const double xpf = xf / A.pitch, zpf = zf / (2.0 * A.pitch);
-        const double clamp_zf = sqrt(zf / (2.88 * A.c0));
+    floatx4 K1[ADJ_ET], K2[ADJ_ET], K3[ADJ_ET];
+#pragma unroll
+    for (int mt = 0; mt < ADJ_ET; ++mt) {
+        K1[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+        K2[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+        K3[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
 
-        // adjoint layout: voxels base + 4s + q, elements cidx + 16 mt
-        uint32_t Qrx[4][ADJ_ET], Qtx[4];
-        float dg[4], yr[4], yi[4];
+    for (long base = vr0; base < vr1; base += 16) {
+        __syncthreads();  // the previous group's tables are consumed
+        // receive phase words of the group: 16 voxels x 128 elements, 4 per thread
+        for (int p = tid; p < 16 * 128; p += 64 * ADJ_WAVES) {
+            const int i = p >> 7, e = p & 127;
+            const long u = min(base + i, n_vox - 1);
+            const int ix = (int)(u / A.n_z), iz = (int)(u - (long)ix * A.n_z);
+            const double x = A.px[ix], z = A.pz[iz];
+            const double lat = (((double)e - A.M) * A.pitch - x) / A.ctof, zc = z / A.ctof;
+            const double tau = sqrt(lat * lat + zc * zc) + (e < A.n_el ? A.delay[e] : 0.0);
+            q_rx[i * ADJ_QS + (e & 15) * ADJ_ET + (e >> 4)] = frac32(A.f_s * tau / A.N);
+        }
+        // transmit phase words: 16 voxels x 16 angle columns of this angle tile
+        for (int p = tid; p < 256; p += 64 * ADJ_WAVES) {
+            const int i = p >> 4, a = 16 * at + (p & 15);
+            const long u = min(base + i, n_vox - 1);
+            const int ix = (int)(u / A.n_z), iz = (int)(u - (long)ix * A.n_z);
+            double tau = 0.0;
+            if (a < A.n_tx) {
+                const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
+                tau = (ex * (A.px[ix] - xr) + ez * A.pz[iz]) / A.ctof + A.t0 / A.f_s;
+            }
+            q_tx[p] = frac32(A.f_s * tau / A.N);
+        }
+        if (tid < 16) {
+            const long u = base + tid;
+            const long uu = min(u, n_vox - 1);
+            const int ix = (int)(uu / A.n_z), iz = (int)(uu - (long)ix * A.n_z);
+            vxz[2 * tid] = A.px[ix];
+            vxz[2 * tid + 1] = A.pz[iz];
+            vy[tid] = u < vr1 ? S.y[u] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        if (!active) continue;
+
+        // aperture of this wave's bin for voxel cidx of the group (forward layout)
+        const double xf = vxz[2 * cidx], zf = vxz[2 * cidx + 1];
+        double Fr = Fb;
+        if (A.clamp) Fr = fmin(Fr, sqf * sqrt(zf / (2.88 * A.c0)));  // das_pw.m:374-377
+        const Aperture ap = make_aperture(xf, xf / A.pitch, zf / (2.0 * A.pitch), Fr, A, A.support_rx);
+        const bool a_ok = 16 * at + cidx < A.n_tx;
+
+        // Gauss: conj(B) U = (br - j bi)(ur + j ui):
+        //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = -bi (ur + ui); Re = K1 - K3, Im = K1 + K2
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const long u = base + 4 * s + q;
-            const bool ok = u < vr1;
-            const long uu = ok ? u : n_vox - 1;
-            const int ix = (int)(uu / A.n_z), iz = (int)(uu - (long)ix * A.n_z);
-            const double x = A.px[ix], z = A.pz[iz];
-            const float2 yv = S.y[uu];  // uu is clamped into the image
-            const bool use = ok && a_lane < A.n_tx;
-            yr[s] = use ? yv.x : 0.f;
-            yi[s] = use ? yv.y : 0.f;
-            const double zc = z / A.ctof;
+            const int i = 4 * s + q;  // voxel of this lane in step s
+            const int klo = __shfl(ap.klo, i), khi = __shfl(ap.khi, i);
+            const float iwl = __shfl(ap.inv_wl, i), iwr = __shfl(ap.inv_wr, i);
+            const float dg = (float)(((double)cidx - A.M) * A.pitch - vxz[2 * i]);
+            float wv[ADJ_ET];
+            float asum = 0.f;
 #pragma unroll
             for (int mt = 0; mt < ADJ_ET; ++mt) {
                 const int e = cidx + 16 * mt;
-                const double lat = (((double)e - A.M) * A.pitch - x) / A.ctof;
-                const double tau = sqrt(lat * lat + zc * zc) + (e < A.n_el ? A.delay[e] : 0.0);
-                Qrx[s][mt] = frac32(A.f_s * tau / A.N);
+                const float d = fmaf((float)(16 * mt), pf, dg);
+                const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
+                float w = window_eval<WCR>(t, A.wrx);
+                w = (e >= klo && e <= khi) ? w : 0.f;
+                wv[mt] = w;
+                asum += w;
             }
-            const double ttx = (ex * (x - xr) + ez * z) / A.ctof + A.t0 / A.f_s;
-            Qtx[s] = frac32(A.f_s * ttx / A.N);
-            dg[s] = (float)(((double)cidx - A.M) * A.pitch - x);
-        }
-
-        for (int c = 0; c < nb; ++c) {
-            const int b = b0 + c;
-            const uint32_t n = (uint32_t)(A.n_lb + b);
-            double Fr = A.Frx[b];
-            if (A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_zf);  // das_pw.m:374-377
-            const Aperture ap = make_aperture(xf, xpf, zpf, Fr, A, A.support_rx);
-
-            // Gauss: conj(B) U = (br - j bi)(ur + j ui):
-            //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = -bi (ur + ui); Re = K1 - K3, Im = K1 + K2
-            floatx4 K1[ADJ_ET], K2[ADJ_ET], K3[ADJ_ET];
+            // weight sum over the 128 elements of voxel i (16 lanes x 8 tiles)
+            asum += __shfl_xor(asum, 1);
+            asum += __shfl_xor(asum, 2);
+            asum += __shfl_xor(asum, 4);
+            asum += __shfl_xor(asum, 8);
+            const float sc = (klo <= khi && a_ok) ? A.out_scale * (A.norm_rx ? 1.0f / asum : 1.0f) : 0.f;
+            // U = s y conj(T) / w
+            const float2 yv = vy[i];
+            const float rt = rev_of(n * q_tx[i * 16 + cidx]);
+            const float st = __builtin_amdgcn_sinf(rt), ct = __builtin_amdgcn_cosf(rt);
+            const float ur = sc * (yv.x * ct + yv.y * st);
+            const float ui = sc * (yv.y * ct - yv.x * st);
+            const float u2 = ui - ur, u3 = ur + ui;
+            const uint4 qa = *reinterpret_cast<const uint4 *>(&q_rx[i * ADJ_QS + cidx * ADJ_ET]);
+            const uint4 qb = *reinterpret_cast<const uint4 *>(&q_rx[i * ADJ_QS + cidx * ADJ_ET + 4]);
+            const uint32_t Q[ADJ_ET] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
 #pragma unroll
             for (int mt = 0; mt < ADJ_ET; ++mt) {
-                K1[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-                K2[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-                K3[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+                const float rv = rev_of(n * Q[mt]);
+                const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                const float br = wv[mt] * cs, bi = wv[mt] * sn;
+                K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br - bi, ur, K1[mt], 0, 0, 0);
+                K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, u2, K2[mt], 0, 0, 0);
+                K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bi, u3, K3[mt], 0, 0, 0);
             }
-            // one voxel step (4 voxels x 128 elements) at a time: weights, their
-            // sum, U, then the MFMAs of the step
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int src = 4 * s + q;
-                const int klo = __shfl(ap.klo, src), khi = __shfl(ap.khi, src);
-                const float iwl = __shfl(ap.inv_wl, src), iwr = __shfl(ap.inv_wr, src);
-                float wv[ADJ_ET];
-                float asum = 0.f;
-#pragma unroll
-                for (int mt = 0; mt < ADJ_ET; ++mt) {
-                    const int e = cidx + 16 * mt;
-                    const float d = fmaf((float)(16 * mt), pf, dg[s]);
-                    const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
-                    float w = window_eval<WCR>(t, A.wrx);
-                    w = (e >= klo && e <= khi) ? w : 0.f;
-                    wv[mt] = w;
-                    asum += w;
-                }
-                // weight sum over the 128 elements of voxel 4s + q (16 lanes x 8 tiles)
-                asum += __shfl_xor(asum, 1);
-                asum += __shfl_xor(asum, 2);
-                asum += __shfl_xor(asum, 4);
-                asum += __shfl_xor(asum, 8);
-                const bool valid = klo <= khi;
-                const float sc = valid ? A.out_scale * (A.norm_rx ? 1.0f / asum : 1.0f) : 0.f;
-                // U = s y conj(T) / w
-                const float rt = rev_of(n * Qtx[s]);
-                const float st = __builtin_amdgcn_sinf(rt), ct = __builtin_amdgcn_cosf(rt);
-                const float ur = sc * (yr[s] * ct + yi[s] * st);
-                const float ui = sc * (yi[s] * ct - yr[s] * st);
-                const float u2 = ui - ur, u3 = ur + ui;
-#pragma unroll
-                for (int mt = 0; mt < ADJ_ET; ++mt) {
-                    const float rv = rev_of(n * Qrx[s][mt]);
-                    const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                    const float br = wv[mt] * cs, bi = wv[mt] * sn;
-                    K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br - bi, ur, K1[mt], 0, 0, 0);
-                    K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, u2, K2[mt], 0, 0, 0);
-                    K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bi, u3, K3[mt], 0, 0, 0);
-                }
-            }
-            // lane holds D[4q + r][cidx] = Z(element 16 mt + 4q + r, angle cidx) of this wave's voxels
-            float *zc_re = zacc + (size_t)(c * 2 + 0) * ADJ_ET * ADJ_TILE;
-            float *zc_im = zacc + (size_t)(c * 2 + 1) * ADJ_ET * ADJ_TILE;
-#pragma unroll
-            for (int mt = 0; mt < ADJ_ET; ++mt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int o = mt * ADJ_TILE + (4 * q + r) * ADJ_PAD + cidx;
-                    atomicAdd(&zc_re[o], K1[mt][r] - K3[mt][r]);
-                    atomicAdd(&zc_im[o], K1[mt][r] + K2[mt][r]);
-                }
         }
     }
-    __syncthreads();
-    // partial of this (range, bin chunk, angle tile): [bin][plane][element 128][angle 16]
-    const int per_bin = 2 * ADJ_ET * 16 * 16;
-    for (int i = threadIdx.x; i < nb * per_bin; i += 64 * ADJ_WAVES) {
-        const int c = i / per_bin, rem = i - c * per_bin;
-        const int plane = rem / (ADJ_ET * 256), r2 = rem - plane * (ADJ_ET * 256);
-        const int mt = r2 >> 8, row = (r2 >> 4) & 15, col = r2 & 15;
-        const float val = zacc[(size_t)(c * 2 + plane) * ADJ_ET * ADJ_TILE + mt * ADJ_TILE + row * ADJ_PAD + col];
-        const size_t dst = (((size_t)blockIdx.x * A.n_f + (b0 + c)) * S.n_at + at) * per_bin + rem;
-        S.P[dst] = val;
-    }
+    if (!active) return;
+    // lane holds D[4q + r][cidx] = Z(element 16 mt + 4q + r, angle column cidx) of the range;
+    // partial layout [range][bin][angle tile][plane][element 128][angle 16]
+    const int per_bin = 2 * ADJ_ET * 256;
+    float *dst = S.P + (((size_t)blockIdx.x * A.n_f + b) * S.n_at + at) * per_bin;
+#pragma unroll
+    for (int mt = 0; mt < ADJ_ET; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int o = (16 * mt + 4 * q + r) * 16 + cidx;
+            dst[o] = K1[mt][r] - K3[mt][r];
+            dst[ADJ_ET * 256 + o] = K1[mt][r] + K2[mt][r];
+        }
 }
 
 // Z[b][a][k] = sum over voxel ranges of the partials (one thread per output)
@@ -290,15 +281,11 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restr
 
 hipError_t launch_adjoint(int wcr, dim3 grid, hipStream_t s, const AdjArgs &S)
 {
-    const void *k = wcr == WC_RANGE ? reinterpret_cast<const void *>(adjoint_kernel<WC_RANGE>)
-                  : wcr == WC_LINEAR ? reinterpret_cast<const void *>(adjoint_kernel<WC_LINEAR>)
-                                     : reinterpret_cast<const void *>(adjoint_kernel<WC_COS>);
-    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ADJ_LDS);
-    if (e != hipSuccess) return e;
+    const dim3 block(64 * ADJ_WAVES);
     switch (wcr) {
-    case WC_RANGE: hipLaunchKernelGGL(adjoint_kernel<WC_RANGE>, grid, dim3(64 * ADJ_WAVES), ADJ_LDS, s, S); break;
-    case WC_LINEAR: hipLaunchKernelGGL(adjoint_kernel<WC_LINEAR>, grid, dim3(64 * ADJ_WAVES), ADJ_LDS, s, S); break;
-    default: hipLaunchKernelGGL(adjoint_kernel<WC_COS>, grid, dim3(64 * ADJ_WAVES), ADJ_LDS, s, S); break;
+    case WC_RANGE: hipLaunchKernelGGL(adjoint_kernel<WC_RANGE>, grid, block, 0, s, S); break;
+    case WC_LINEAR: hipLaunchKernelGGL(adjoint_kernel<WC_LINEAR>, grid, block, 0, s, S); break;
+    default: hipLaunchKernelGGL(adjoint_kernel<WC_COS>, grid, block, 0, s, S); break;
     }
     return hipGetLastError();
 }

@@ -554,13 +554,14 @@ int execute_tangent(fn_plan *P, const void *d_rf, int dtype, float *d_image, flo
 int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
 {
     const int n_f = P->g.n_f;
-    const int n_chunks = (n_f + 7) / 8;  // bins per block of the contraction kernel
+    const int n_chunks = (n_f + 7) / 8;  // bins per block of the contraction kernel (one per wave)
     if (!P->d_Z) {
         P->adj_at = (P->n_tx + 15) / 16;
         const long groups16 = (P->n_vox + 15) / 16;
-        // enough blocks for 256 CUs (one block per CU at a time), bounded by the voxel groups
-        long vr = (512 + (long)n_chunks * P->adj_at - 1) / ((long)n_chunks * P->adj_at);
-        vr = std::max(1L, std::min(vr, (groups16 + 3) / 4));
+        // about two rounds of one block per CU on 256 CUs (rounded down so the
+        // last round is not a short tail), bounded by the voxel groups
+        long vr = 512 / ((long)n_chunks * P->adj_at);
+        vr = std::max(1L, std::min(vr, groups16));
         const long groups_per_range = (groups16 + vr - 1) / vr;
         P->adj_vpr = groups_per_range * 16;
         P->adj_vr = (int)((groups16 + groups_per_range - 1) / groups_per_range);
@@ -765,6 +766,36 @@ int32_t fn_das_pw(const fn_pw_params *p, const void *data_rf, int32_t rf_dtype, 
     int s = pw_plan(p, &P);
     if (s) return s;
     s = one_shot(P, data_rf, rf_dtype, image, weights, nullptr);
+    destroy(P);
+    return s;
+}
+
+int32_t fn_das_pw_adjoint(const fn_pw_params *p, double c_tof, const double *delay_offsets, const float *image_y,
+                          float *rf_adj)
+{
+    fnh::clear_error();
+    if (!image_y || !rf_adj) return fail(FN_ERR_ARGUMENT, "image and output buffers are required");
+    fn_plan *P = nullptr;
+    int s = pw_plan(p, &P);
+    if (s) return s;
+    s = set_tof(P, c_tof, delay_offsets);
+    const size_t img_bytes = (size_t)P->n_vox * 2 * sizeof(float);
+    const size_t rf_bytes = (size_t)P->n_t * P->n_el * P->n_tx * sizeof(float);
+    float *d_y = nullptr, *d_rf = nullptr;
+    if (!s) {
+        hipError_t e = hipMalloc(&d_y, img_bytes);
+        if (e == hipSuccess) e = hipMalloc(&d_rf, rf_bytes);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_y, image_y, img_bytes, hipMemcpyHostToDevice, P->stream);
+        if (e != hipSuccess) s = hip_fail(e, "adjoint buffers");
+    }
+    if (!s) s = adjoint(P, d_y, d_rf, P->stream);
+    if (!s) {
+        hipError_t e = hipMemcpyAsync(rf_adj, d_rf, rf_bytes, hipMemcpyDeviceToHost, P->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+        if (e != hipSuccess) s = hip_fail(e, "adjoint result");
+    }
+    hipFree(d_y);
+    hipFree(d_rf);
     destroy(P);
     return s;
 }

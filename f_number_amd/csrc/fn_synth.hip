@@ -8,6 +8,11 @@
 // fractional_bandwidth, placed at the exact transmit + receive delay:
 //   plane wave a:  tau = (e_x (x_s - x_ref) + e_z z_s) / c + |r_k - r_s| / c   (das_pw.m:336,427)
 //   element i -> k: tau = |r_i - r_s| / c + |r_k - r_s| / c                     (das_sa.m:431)
+// Layered medium (straight rays): a path of length L from the surface to depth
+// z_s spends the share f = min(1, h / z_s) of its length in the layer, so its
+// time is L / c_0 + f L (1 / c_layer - 1 / c_0); the plane-wave transmit path to
+// depth z_s crosses the layer over h / e_z.  Receive delay offsets add per element.
+// Flow: scatterers inside the tube move along its axis (parabolic profile).
 // White Gaussian noise at snr_db relative to the RMS of the clean signal.
 // Random numbers: splitmix64 (Steele, Lea, Flood, "Fast splittable
 // pseudorandom number generators", OOPSLA 2014), stateless per noise sample.
@@ -45,22 +50,34 @@ __host__ __device__ inline double gauss_at(uint64_t seed, uint64_t i)
 struct Geo {
     int n_t, n_el, n_tx, mode, n_s;
     double f_s, pitch, c0, M, fc, sigma, half;  // half: pulse half-support (s)
+    double h, slow;                              // layer thickness, 1/c_layer - 1/c_0 (0: homogeneous)
+    const double *delay;                         // per-element receive offsets (device or host), may be null
 };
+
+// time along a straight path of length L from the surface to depth zs
+__host__ __device__ inline double path_time(const Geo &G, double L, double zs)
+{
+    double t = L / G.c0;
+    if (G.h > 0.0) t += L * fmin(1.0, G.h / zs) * G.slow;
+    return t;
+}
 
 __host__ __device__ inline double tx_delay(const Geo &G, const double *txp, int a, double xs, double zs)
 {
     if (G.mode == 0) {
         const double ex = txp[3 * a], ez = txp[3 * a + 1], xr = txp[3 * a + 2];
-        return (ex * (xs - xr) + ez * zs) / G.c0;
+        double t = (ex * (xs - xr) + ez * zs) / G.c0;
+        if (G.h > 0.0) t += fmin(zs, G.h) / ez * G.slow;
+        return t;
     }
     const double dx = ((double)a - G.M) * G.pitch - xs;
-    return sqrt(dx * dx + zs * zs) / G.c0;
+    return path_time(G, sqrt(dx * dx + zs * zs), zs);
 }
 
 __host__ __device__ inline double rx_delay(const Geo &G, int k, double xs, double zs)
 {
     const double dx = ((double)k - G.M) * G.pitch - xs;
-    return sqrt(dx * dx + zs * zs) / G.c0;
+    return path_time(G, sqrt(dx * dx + zs * zs), zs) + (G.delay ? G.delay[k] : 0.0);
 }
 
 __host__ __device__ inline double pulse(const Geo &G, double t)
@@ -148,6 +165,15 @@ int build_geo(const fn_synth_params *p, Geo *G, std::vector<double> *txp)
     G->pitch = p->element_pitch;
     G->c0 = p->c_0;
     G->M = (p->n_elements - 1) / 2.0;
+    G->h = 0.0;
+    G->slow = 0.0;
+    G->delay = nullptr;
+    if (p->layer_thickness > 0.0) {
+        if (!(p->layer_c > 0.0)) return fail(FN_ERR_ARGUMENT, "layer_c must be positive");
+        G->h = p->layer_thickness;
+        G->slow = 1.0 / p->layer_c - 1.0 / p->c_0;
+    }
+    if (p->tube_radius < 0.0) return fail(FN_ERR_ARGUMENT, "tube_radius must be >= 0");
     G->fc = p->f_c;
     // -6 dB half-bandwidth bw f_c / 2 = sigma_f sqrt(2 ln 2), sigma = 1 / (2 pi sigma_f)
     const double sigma_f = p->fractional_bandwidth * p->f_c / (2.0 * std::sqrt(2.0 * std::log(2.0)));
@@ -195,6 +221,21 @@ void scatterers(const fn_synth_params *p, double *xs, double *zs, double *amp)
             amp[s] = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
         }
     }
+    // flow: axial coordinate u along the tube, radial w; v(w) = v_max (1 - (w/R)^2),
+    // u wrapped over [-W/2, W/2) with W the lateral extent of the field
+    if (p->tube_radius > 0.0 && p->frame_time != 0.0) {
+        const double ca = std::cos(p->tube_angle), sa = std::sin(p->tube_angle);
+        const double W = p->x_max - p->x_min, R = p->tube_radius;
+        for (int s = 0; s < n; ++s) {
+            const double dx = xs[s] - p->tube_x, dz = zs[s] - p->tube_z;
+            const double u = dx * ca + dz * sa, w = -dx * sa + dz * ca;
+            if (std::fabs(w) >= R) continue;
+            double u2 = u + p->tube_v_max * (1.0 - (w / R) * (w / R)) * p->frame_time;
+            u2 -= W * std::floor(u2 / W + 0.5);
+            xs[s] = p->tube_x + u2 * ca - w * sa;
+            zs[s] = p->tube_z + u2 * sa + w * ca;
+        }
+    }
 }
 
 }  // namespace
@@ -220,11 +261,14 @@ int32_t fn_synth_rf(const fn_synth_params *p, float *rf)
     if (s) return s;
     std::vector<double> xs(G.n_s), zs(G.n_s), amp(G.n_s);
     scatterers(p, xs.data(), zs.data(), amp.data());
+    std::vector<double> delays;
+    if (p->delay_offsets) delays.assign(p->delay_offsets, p->delay_offsets + G.n_el);
     const long n_ch = (long)G.n_el * G.n_tx;
     const long n = n_ch * G.n_t;
     const bool noisy = std::isfinite(p->snr_db);
     if (!p->on_device) {
         // host twin of synth_signal_kernel: same delays, same per-sample summation order
+        if (!delays.empty()) G.delay = delays.data();
         std::vector<double> sig((size_t)n, 0.0);
         std::atomic<long> next_ch{0};
         auto worker = [&]() {
@@ -264,9 +308,14 @@ int32_t fn_synth_rf(const fn_synth_params *p, float *rf)
     if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return fail(FN_ERR_NO_DEVICES, "Could not find any HIP devices!");
     if (p->index_gpu < 0 || p->index_gpu >= count) return fail(FN_ERR_DEVICE, "Selected device does not exist!");
     if (hipSetDevice(p->index_gpu) != hipSuccess) return fail(FN_ERR_HIP, "hipSetDevice failed");
-    double *d_x = nullptr, *d_z = nullptr, *d_a = nullptr, *d_t = nullptr, *d_part = nullptr;
+    double *d_x = nullptr, *d_z = nullptr, *d_a = nullptr, *d_t = nullptr, *d_part = nullptr, *d_dl = nullptr;
     const int n_red = 1024;
     hipError_t e = hipMalloc(&d_x, sizeof(double) * std::max(1, G.n_s));
+    if (e == hipSuccess && !delays.empty()) {
+        e = hipMalloc(&d_dl, sizeof(double) * delays.size());
+        if (e == hipSuccess) e = hipMemcpy(d_dl, delays.data(), sizeof(double) * delays.size(), hipMemcpyHostToDevice);
+        G.delay = d_dl;
+    }
     if (e == hipSuccess) e = hipMalloc(&d_z, sizeof(double) * std::max(1, G.n_s));
     if (e == hipSuccess) e = hipMalloc(&d_a, sizeof(double) * std::max(1, G.n_s));
     if (e == hipSuccess) e = hipMalloc(&d_t, sizeof(double) * txp.size());
@@ -293,7 +342,7 @@ int32_t fn_synth_rf(const fn_synth_params *p, float *rf)
         }
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    hipFree(d_x); hipFree(d_z); hipFree(d_a); hipFree(d_t); hipFree(d_part);
+    hipFree(d_x); hipFree(d_z); hipFree(d_a); hipFree(d_t); hipFree(d_part); hipFree(d_dl);
     if (e != hipSuccess) return fail(FN_ERR_HIP, "synthetic RF: %s", hipGetErrorString(e));
     return FN_OK;
 }

@@ -132,6 +132,7 @@ class PwPlan(_Plan):
         self._keep = keep
         self.grid = info
         self.n_samples, self.n_elements, self.n_angles = int(n_samples), int(n_elements), len(keep[2])
+        self.c_0 = float(c_0)
         self._rf_numel = self.n_samples * self.n_elements * self.n_angles
 
     def set_tof(self, c_tof=0.0, delay_offsets=None):

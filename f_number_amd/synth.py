@@ -44,6 +44,13 @@ class Workload:
     scatter_z: tuple = (5e-3, 40e-3)
     snr_db: float = 30.0
     extra: dict = field(default_factory=dict)
+    # medium: (thickness m, speed of sound m/s) of a surface layer; (0, 0) = homogeneous
+    layer: tuple = (0.0, 0.0)
+    # per-element receive delay offsets (s) of the acquisition, or None
+    delay_offsets: np.ndarray | None = None
+    # flow tube (x, z, radius, angle rad, v_max m/s) and the frame time (s)
+    tube: tuple = (0.0, 0.0, 0.0, 0.0, 0.0)
+    frame_time: float = 0.0
 
     @property
     def angles(self) -> np.ndarray:
@@ -75,7 +82,16 @@ class Workload:
         P.steering_angles = ang.ctypes.data_as(C.POINTER(C.c_double))
         P.on_device = int(on_device)
         P.index_gpu = index_gpu
-        return P, ang
+        P.layer_thickness, P.layer_c = float(self.layer[0]), float(self.layer[1])
+        keep = [ang]
+        if self.delay_offsets is not None:
+            d = np.ascontiguousarray(self.delay_offsets, dtype=np.float64)
+            assert d.size == self.n_elements
+            P.delay_offsets = d.ctypes.data_as(C.POINTER(C.c_double))
+            keep.append(d)
+        P.tube_x, P.tube_z, P.tube_radius, P.tube_angle, P.tube_v_max = (float(v) for v in self.tube)
+        P.frame_time = float(self.frame_time)
+        return P, keep
 
     def rf_host(self) -> np.ndarray:
         """RF as float32 [N_tx, N_el, N_t] (C order) = MATLAB's [N_t x N_el x N_tx]."""
@@ -130,6 +146,30 @@ def saft_full() -> Workload:
     x = (np.arange(512) - 255.5) * 0.075e-3
     z = 5e-3 + np.arange(512) * 0.075e-3
     return Workload("saft_128x128_512x512", "sa", 3000, np.zeros(0), x, z, (2.25e6, 6.75e6), seed=20260002)
+
+
+def autofocus_layer() -> Workload:
+    """Differentiation-form acquisition: the 11-angle array of cpwc_small (2048 samples,
+    2.25-6.75 MHz) over a 6 mm surface layer at 1450 m/s on a 1540 m/s background, with
+    per-element receive delay offsets N(0, 20 ns) (seed 20260004 + 2000); 384 x 384 voxels
+    over x in [-19, 19] mm, z in [5, 40] mm."""
+    w = Workload("autofocus_layer_11ang_384x384", "pw", 2048, np.arange(-10.0, 10.0 + 1e-9, 2.0),
+                 _lin(-19e-3, 19e-3, 384), _lin(5e-3, 40e-3, 384), (2.25e6, 6.75e6), seed=20260004)
+    w.layer = (6e-3, 1450.0)
+    w.delay_offsets = np.random.default_rng(w.seed + 2000).normal(0.0, 20e-9, w.n_elements)
+    return w
+
+
+def doppler_frame(frame: int, frame_interval: float = 2e-4) -> Workload:
+    """Ultrafast-Doppler ensemble frame: 5 angles -4..4 deg, 1536 samples, 2.25-6.75 MHz,
+    256 x 256 voxels over x in [-19, 19] mm, z in [5, 40] mm; scatterers inside a 4 mm tube
+    (centre z = 20 mm, axis tilted 10 deg) flow at up to 0.5 m/s (parabolic profile);
+    frame k is the medium at time k * frame_interval (seed 20260003)."""
+    w = Workload("doppler_5ang_256x256", "pw", 1536, np.arange(-4.0, 4.0 + 1e-9, 2.0),
+                 _lin(-19e-3, 19e-3, 256), _lin(5e-3, 40e-3, 256), (2.25e6, 6.75e6), seed=20260003)
+    w.tube = (0.0, 20e-3, 2e-3, np.deg2rad(10.0), 0.5)
+    w.frame_time = frame * frame_interval
+    return w
 
 
 def scaled(w: Workload, n_x: int, n_z: int, n_angles: int | None = None, n_samples: int | None = None,

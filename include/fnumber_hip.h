@@ -251,6 +251,12 @@ int32_t fn_plan_adjoint(fn_plan *plan, const float *d_y, float *d_rf_adj, void *
 int32_t fn_plan_vjp(fn_plan *plan, const void *d_rf, int32_t rf_dtype, const float *d_g,
                     const float *d_dimage_dc, double *d_vjp_delay, double *d_vjp_c, void *stream);
 
+/* One-shot adjoint on host buffers (plans the shape, uploads, runs, copies back):
+ * image_y complex float interleaved [n_x][n_z] -> rf_adj float [n_angles][n_elements][n_samples]
+ * (MATLAB [N_t x N_el x N_a]), with the time-of-flight parameters of fn_plan_set_tof. */
+int32_t fn_das_pw_adjoint(const fn_pw_params *p, double c_tof, const double *delay_offsets,
+                          const float *image_y, float *rf_adj);
+
 /* ---- synthetic acquisitions (seeded; the reference ships no data) ------ */
 typedef struct fn_synth_params {
     uint32_t struct_size;
@@ -267,6 +273,15 @@ typedef struct fn_synth_params {
     const double *steering_angles;
     int32_t on_device;             /* 1: rf is a device pointer, computed on the GPU */
     int32_t index_gpu;
+    /* medium: a layer 0 <= z < layer_thickness at layer_c over c_0 (0: homogeneous);
+     * straight rays, the layer's share of a path is min(1, layer_thickness / z_s) */
+    double layer_thickness, layer_c;
+    const double *delay_offsets;   /* per-element receive delay offsets (s), NULL = 0 */
+    /* flow: scatterers within tube_radius of the axis through (tube_x, tube_z) at
+     * tube_angle (rad from +x) move along it with v = tube_v_max (1 - (r/R)^2),
+     * displaced by v * frame_time and wrapped over the lateral extent; R = 0: none */
+    double tube_x, tube_z, tube_radius, tube_angle, tube_v_max;
+    double frame_time;
 } fn_synth_params;
 /* rf: float32 column-major [n_samples x n_elements x (n_angles | n_elements)] */
 int32_t fn_synth_rf(const fn_synth_params *p, float *rf);

@@ -111,6 +111,19 @@ mxArray *mxCreateNumericMatrix(mwSize m, mwSize n, mxClassID cls, mxComplexity c
     a->complex_ = c == mxCOMPLEX;
     return a;
 }
+mxArray *mxCreateNumericArray(mwSize ndim, const mwSize *dims, mxClassID cls, mxComplexity c)
+{
+    mxArray *a = mxCreateNumericMatrix(0, 0, cls, c);
+    a->dims.assign(dims, dims + ndim);
+    const size_t n = numel(a);
+    if (cls == mxSINGLE_CLASS) {
+        a->fr.assign(n, 0.f);
+        if (c == mxCOMPLEX) a->fi.assign(n, 0.f);
+    } else {
+        a->pr.assign(n, 0.0);
+    }
+    return a;
+}
 mxArray *mxDuplicateArray(const mxArray *a) { return new mxArray_tag(*a); }
 void mxDestroyArray(mxArray *a) { delete a; }
 
@@ -235,6 +248,16 @@ void fm_get_single(const mxArray *a, float *re, float *im)
 {
     std::memcpy(re, a->fr.data(), a->fr.size() * sizeof(float));
     if (im && !a->fi.empty()) std::memcpy(im, a->fi.data(), a->fi.size() * sizeof(float));
+}
+int fm_is_double(const mxArray *a) { return a->cls == "double"; }
+void fm_get_double(const mxArray *a, double *out) { std::memcpy(out, a->pr.data(), a->pr.size() * sizeof(double)); }
+// complex single matrix [m x n] from separate parts (column-major)
+mxArray *fm_single_complex(const float *re, const float *im, size_t m, size_t n)
+{
+    mxArray *a = mxCreateNumericMatrix(m, n, mxSINGLE_CLASS, mxCOMPLEX);
+    std::memcpy(a->fr.data(), re, m * n * sizeof(float));
+    std::memcpy(a->fi.data(), im, m * n * sizeof(float));
+    return a;
 }
 void fm_free(mxArray *a) { delete a; }
 }

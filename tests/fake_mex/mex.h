@@ -34,6 +34,7 @@ mxArray *mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c);
 mxArray *mxCreateDoubleScalar(double v);
 mxArray *mxCreateString(const char *s);
 mxArray *mxCreateNumericMatrix(mwSize m, mwSize n, mxClassID cls, mxComplexity c);
+mxArray *mxCreateNumericArray(mwSize ndim, const mwSize *dims, mxClassID cls, mxComplexity c);
 mxArray *mxDuplicateArray(const mxArray *a);
 void mxDestroyArray(mxArray *a);
 int mexCallMATLAB(int nlhs, mxArray *plhs[], int nrhs, mxArray *prhs[], const char *name);

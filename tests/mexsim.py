@@ -34,6 +34,10 @@ def load(which):
     L.fm_dim.argtypes = [vp, C.c_size_t]
     L.fm_is_single_complex.argtypes = [vp]
     L.fm_get_single.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.fm_is_double.argtypes = [vp]
+    L.fm_get_double.argtypes = [vp, C.POINTER(C.c_double)]
+    L.fm_single_complex.restype = vp
+    L.fm_single_complex.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t, C.c_size_t]
     L.fm_free.argtypes = [vp]
     return Gateway(L)
 
@@ -68,6 +72,14 @@ class Gateway:
     def normalization(self, name):
         return self.obj(f"normalizations.{name}", "normalizations.normalization")
 
+    def single_complex(self, a):
+        """complex single matrix (MATLAB layout) from a 2-D numpy array"""
+        a = np.asarray(a)
+        re = np.ascontiguousarray(a.real.reshape(-1, order="F"), dtype=np.float32)
+        im = np.ascontiguousarray(a.imag.reshape(-1, order="F"), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        return self.L.fm_single_complex(re.ctypes.data_as(fp), im.ctypes.data_as(fp), a.shape[0], a.shape[1])
+
     def empty(self):
         return self.L.fm_empty()
 
@@ -85,6 +97,13 @@ class Gateway:
         for i in range(nlhs):
             h = plhs[i]
             n = self.L.fm_numel(h)
+            if self.L.fm_is_double(h):
+                d = np.empty(n, dtype=np.float64)
+                self.L.fm_get_double(h, d.ctypes.data_as(C.POINTER(C.c_double)))
+                shape = tuple(self.L.fm_dim(h, j) for j in range(3))
+                outs.append(d.reshape(shape, order="F"))
+                self.L.fm_free(h)
+                continue
             re = np.empty(n, dtype=np.float32)
             im = np.empty(n, dtype=np.float32)
             cplx = self.L.fm_is_single_complex(h)

@@ -123,3 +123,70 @@ def test_sa_gateway_matches_python_api(sa):
     bad = list(args)
     bad[2] = sa.double(np.zeros((64, 4, 5)))
     assert sa.call(1, bad)[1] == "gpu_bf_saft_rf:InvalidRFData"
+
+
+@pytest.fixture(scope="module")
+def pwadj():
+    mexsim.build()
+    return mexsim.load("pwadj")
+
+
+def test_adjoint_gateway_reports_missing_device_first(pwadj):
+    if _devices() > 0:
+        pytest.skip("a GPU is present")
+    out, err = pwadj.call(1, [pwadj.empty()] * 9)
+    assert out is None and err == "gpu_bf_das_pw_rf_adjoint:NoCUDADevices"
+
+
+@pytest.mark.gpu
+def test_adjoint_gateway_is_the_adjoint_of_the_forward_gateway(pw, pwadj):
+    """<A x, y> = <x, A^H y> across the two MEX gateways (MEX output scale, single
+    precision outputs, sums in FP64), with delay offsets and c_tof through the adjoint
+    gateway and the Python plan API for the matching forward."""
+    import torch
+
+    import f_number_amd as fa
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.cpwc_small(), 12, 10, n_angles=3, n_samples=768, n_speckle=600, n_elements=32)
+    rf = w.rf_host()
+    rf_m = np.transpose(rf, (2, 1, 0)).astype(np.float64)
+    common = [pw.double(w.f_s), pw.double(w.angles), pw.double(w.width), pw.double(w.pitch), pw.double(w.c_0),
+              pw.double(list(w.f_bounds)), pw.double(0.0), pw.window("hann"),
+              pw.fnumber("f_numbers.grating.angle_lb", angle_lb_deg=60.0, F_number_ub=1.5, distance_deg=10.0),
+              pw.normalization("on"), pw.double(0.0), pw.double(0.0)]
+    out, err = pw.call(1, [pw.double(w.positions_x), pw.double(w.positions_z), pw.double(rf_m)] + common)
+    assert err is None
+    Ax = out[0].astype(np.complex128)
+    rng = np.random.default_rng(2)
+    y = (Ax + 0.1 * np.abs(Ax).mean() * (rng.normal(size=Ax.shape) + 1j * rng.normal(size=Ax.shape)))
+    y = y.astype(np.complex64)
+    g = pwadj
+    cargs = [g.double(w.f_s), g.double(w.angles), g.double(w.width), g.double(w.pitch), g.double(w.c_0),
+             g.double(list(w.f_bounds)), g.double(0.0), g.window("hann"),
+             g.fnumber("f_numbers.grating.angle_lb", angle_lb_deg=60.0, F_number_ub=1.5, distance_deg=10.0),
+             g.normalization("on"), g.double(0.0), g.double(0.0)]
+    out, err = g.call(1, [g.double(w.positions_x), g.double(w.positions_z), g.single_complex(y),
+                          g.double([w.n_samples, w.n_elements])] + cargs)
+    assert err is None
+    AHy = out[0]
+    assert AHy.shape == (w.n_samples, w.n_elements, 3)
+    lhs = np.real(np.vdot(Ax, y.astype(np.complex128)))
+    rhs = np.sum(rf_m * AHy)
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+    # with delay offsets and c_tof: matches the plan API's adjoint (MEX scale)
+    delays = rng.normal(0, 20e-9, w.n_elements)
+    out, err = g.call(1, [g.double(w.positions_x), g.double(w.positions_z), g.single_complex(y),
+                          g.double([w.n_samples, w.n_elements])] + cargs +
+                      [g.double(1510.0), g.double(delays)])
+    assert err is None
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     w.c_0, w.f_bounds, 0.0, window=fa.windows.hann(), F_number=fa.f_numbers.grating.angle_lb(60, 1.5),
+                     normalization=fa.normalizations.on(), output_scale="mex")
+    plan.set_tof(1510.0, delays)
+    yd = torch.from_numpy(np.ascontiguousarray(y.T)).cuda()
+    ref = plan.adjoint(yd)
+    torch.cuda.synchronize()
+    ref = np.transpose(ref.cpu().numpy(), (2, 1, 0))
+    plan.close()
+    assert rel_l2(out[0], ref) < 1e-6

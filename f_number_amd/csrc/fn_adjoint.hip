@@ -34,6 +34,17 @@
 
 namespace fnk {
 
+// sum over the 16 lanes of a DPP row, result in every lane of the row:
+// quad xor 1, quad xor 2, half-row mirror (quad <-> quad), row mirror (half <-> half)
+__device__ __forceinline__ float row_sum16(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, true));  // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, true));  // row_mirror
+    return v;
+}
+
 constexpr int ADJ_WAVES = 8;   // bins per block, one per wave
 constexpr int ADJ_ET = 8;      // element tiles of 16 (128 elements)
 constexpr int ADJ_QS = 132;    // LDS words per voxel of the receive phase table ([cidx][mt] + pad)
@@ -41,10 +52,12 @@ constexpr int ADJ_QS = 132;    // LDS words per voxel of the receive phase table
 template <int WCR>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(AdjArgs S)
 {
-    __shared__ __align__(16) uint32_t q_rx[16 * ADJ_QS];  // [voxel i][cidx][mt]: element cidx + 16 mt
-    __shared__ uint32_t q_tx[16 * 16];      // [voxel i][angle column]
-    __shared__ double vxz[2 * 16];          // x, z of the group's voxels
-    __shared__ float2 vy[16];               // cotangent of the group's voxels (0 outside the range)
+    // group tables, double-buffered: the next group is loaded while this one is contracted
+    __shared__ __align__(16) uint32_t q_rx[2][16 * ADJ_QS];  // [voxel i][cidx][mt]: element cidx + 16 mt
+    __shared__ uint32_t q_tx[2][16 * 16];      // [voxel i][angle column]
+    __shared__ double vxz[2][2 * 16];          // x, z of the group's voxels
+    __shared__ float2 vy[2][16];               // cotangent of the group's voxels (0 outside the range)
+    __shared__ __align__(16) uint4 apx[ADJ_WAVES][16];  // per wave: aperture (klo, khi, 1/W_L, 1/W_R) per voxel
     const BeamArgs &A = S.B;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cidx = lane & 15, q = lane >> 4;
@@ -70,94 +83,92 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void a
         K3[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
 
-    for (long base = vr0; base < vr1; base += 16) {
-        __syncthreads();  // the previous group's tables are consumed
-        // receive phase words of the group: 16 voxels x 128 elements, 4 per thread
-        for (int p = tid; p < 16 * 128; p += 64 * ADJ_WAVES) {
-            const int i = p >> 7, e = p & 127;
-            const long u = min(base + i, n_vox - 1);
-            const int ix = (int)(u / A.n_z), iz = (int)(u - (long)ix * A.n_z);
-            const double x = A.px[ix], z = A.pz[iz];
-            const double lat = (((double)e - A.M) * A.pitch - x) / A.ctof, zc = z / A.ctof;
-            const double tau = sqrt(lat * lat + zc * zc) + (e < A.n_el ? A.delay[e] : 0.0);
-            q_rx[i * ADJ_QS + (e & 15) * ADJ_ET + (e >> 4)] = frac32(A.f_s * tau / A.N);
+    const unsigned nz = (unsigned)A.n_z;
+    const int qtx_stride = 16 * S.n_at;
+    // copy the phase words and voxel data of the group at `base` into buffer `buf`
+    auto load_group = [&](long base, int buf) {
+        const int i = tid >> 5, j4 = tid & 31;  // 512 threads: 16 voxels x 32 uint4
+        const long u = base + i;
+        uint4 w = make_uint4(0u, 0u, 0u, 0u);
+        if (u < n_vox) w = reinterpret_cast<const uint4 *>(S.qrx + (size_t)u * 128)[j4];
+        *reinterpret_cast<uint4 *>(&q_rx[buf][i * ADJ_QS + 4 * j4]) = w;
+        if (tid < 256) {
+            const int iv = tid >> 4, col = tid & 15;
+            const long uv = base + iv;
+            q_tx[buf][tid] = uv < n_vox ? S.qtx[(size_t)uv * qtx_stride + 16 * at + col] : 0u;
+        } else if (tid < 272) {
+            const int t = tid - 256;
+            const long u2 = base + t;
+            const unsigned uu = (unsigned)min(u2, n_vox - 1);  // n_vox < 2^31
+            const unsigned ix = uu / nz, iz = uu - ix * nz;
+            vxz[buf][2 * t] = A.px[ix];
+            vxz[buf][2 * t + 1] = A.pz[iz];
+            vy[buf][t] = u2 < vr1 ? S.y[u2] : make_float2(0.f, 0.f);
         }
-        // transmit phase words: 16 voxels x 16 angle columns of this angle tile
-        for (int p = tid; p < 256; p += 64 * ADJ_WAVES) {
-            const int i = p >> 4, a = 16 * at + (p & 15);
-            const long u = min(base + i, n_vox - 1);
-            const int ix = (int)(u / A.n_z), iz = (int)(u - (long)ix * A.n_z);
-            double tau = 0.0;
-            if (a < A.n_tx) {
-                const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
-                tau = (ex * (A.px[ix] - xr) + ez * A.pz[iz]) / A.ctof + A.t0 / A.f_s;
-            }
-            q_tx[p] = frac32(A.f_s * tau / A.N);
-        }
-        if (tid < 16) {
-            const long u = base + tid;
-            const long uu = min(u, n_vox - 1);
-            const int ix = (int)(uu / A.n_z), iz = (int)(uu - (long)ix * A.n_z);
-            vxz[2 * tid] = A.px[ix];
-            vxz[2 * tid + 1] = A.pz[iz];
-            vy[tid] = u < vr1 ? S.y[u] : make_float2(0.f, 0.f);
-        }
-        __syncthreads();
-        if (!active) continue;
+    };
+    if (vr0 < vr1) load_group(vr0, 0);
+    __syncthreads();
+    int cur = 0;
+    for (long base = vr0; base < vr1; base += 16, cur ^= 1) {
+        if (base + 16 < vr1) load_group(base + 16, cur ^ 1);  // the buffer last read two groups ago
+        if (active) {
+            // aperture of this wave's bin for voxel cidx of the group (forward layout)
+            const double xf = vxz[cur][2 * cidx], zf = vxz[cur][2 * cidx + 1];
+            double Fr = Fb;
+            if (A.clamp) Fr = fmin(Fr, sqf * sqrt(zf / (2.88 * A.c0)));  // das_pw.m:374-377
+            const Aperture ap = make_aperture(xf, xf / A.pitch, zf / (2.0 * A.pitch), Fr, A, A.support_rx);
+            const bool a_ok = 16 * at + cidx < A.n_tx;
+            // exchange through LDS: lanes of row q need the apertures of voxels 4s + q
+            if (lane < 16)
+                apx[wave][cidx] = make_uint4((uint32_t)ap.klo, (uint32_t)ap.khi, __float_as_uint(ap.inv_wl),
+                                             __float_as_uint(ap.inv_wr));
 
-        // aperture of this wave's bin for voxel cidx of the group (forward layout)
-        const double xf = vxz[2 * cidx], zf = vxz[2 * cidx + 1];
-        double Fr = Fb;
-        if (A.clamp) Fr = fmin(Fr, sqf * sqrt(zf / (2.88 * A.c0)));  // das_pw.m:374-377
-        const Aperture ap = make_aperture(xf, xf / A.pitch, zf / (2.0 * A.pitch), Fr, A, A.support_rx);
-        const bool a_ok = 16 * at + cidx < A.n_tx;
-
-        // Gauss: conj(B) U = (br - j bi)(ur + j ui):
-        //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = -bi (ur + ui); Re = K1 - K3, Im = K1 + K2
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int i = 4 * s + q;  // voxel of this lane in step s
-            const int klo = __shfl(ap.klo, i), khi = __shfl(ap.khi, i);
-            const float iwl = __shfl(ap.inv_wl, i), iwr = __shfl(ap.inv_wr, i);
-            const float dg = (float)(((double)cidx - A.M) * A.pitch - vxz[2 * i]);
-            float wv[ADJ_ET];
-            float asum = 0.f;
-#pragma unroll
-            for (int mt = 0; mt < ADJ_ET; ++mt) {
-                const int e = cidx + 16 * mt;
-                const float d = fmaf((float)(16 * mt), pf, dg);
-                const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
-                float w = window_eval<WCR>(t, A.wrx);
-                w = (e >= klo && e <= khi) ? w : 0.f;
-                wv[mt] = w;
-                asum += w;
+            // Gauss: conj(B) U = (br - j bi)(ur + j ui):
+            //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = -bi (ur + ui); Re = K1 - K3, Im = K1 + K2
+    #pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int i = 4 * s + q;  // voxel of this lane in step s
+                const uint4 apv = apx[wave][i];  // written by this wave above (LDS operations of a wave are ordered)
+                const int klo = (int)apv.x, khi = (int)apv.y;
+                const float iwl = __uint_as_float(apv.z), iwr = __uint_as_float(apv.w);
+                const float dg = (float)(((double)cidx - A.M) * A.pitch - vxz[cur][2 * i]);
+                float wv[ADJ_ET];
+                float asum = 0.f;
+    #pragma unroll
+                for (int mt = 0; mt < ADJ_ET; ++mt) {
+                    const int e = cidx + 16 * mt;
+                    const float d = fmaf((float)(16 * mt), pf, dg);
+                    const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
+                    float w = window_eval<WCR>(t, A.wrx);
+                    w = (e >= klo && e <= khi) ? w : 0.f;
+                    wv[mt] = w;
+                    asum += w;
+                }
+                // weight sum over the 128 elements of voxel i: 16 lanes of the row (DPP)
+                asum = row_sum16(asum);
+                const float sc = (klo <= khi && a_ok) ? A.out_scale * (A.norm_rx ? 1.0f / asum : 1.0f) : 0.f;
+                // U = s y conj(T) / w
+                const float2 yv = vy[cur][i];
+                const float rt = rev_of(n * q_tx[cur][i * 16 + cidx]);
+                const float st = __builtin_amdgcn_sinf(rt), ct = __builtin_amdgcn_cosf(rt);
+                const float ur = sc * (yv.x * ct + yv.y * st);
+                const float ui = sc * (yv.y * ct - yv.x * st);
+                const float u2 = ui - ur, u3 = ur + ui;
+                const uint4 qa = *reinterpret_cast<const uint4 *>(&q_rx[cur][i * ADJ_QS + cidx * ADJ_ET]);
+                const uint4 qb = *reinterpret_cast<const uint4 *>(&q_rx[cur][i * ADJ_QS + cidx * ADJ_ET + 4]);
+                const uint32_t Q[ADJ_ET] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+    #pragma unroll
+                for (int mt = 0; mt < ADJ_ET; ++mt) {
+                    const float rv = rev_of(n * Q[mt]);
+                    const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                    const float br = wv[mt] * cs, bi = wv[mt] * sn;
+                    K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br - bi, ur, K1[mt], 0, 0, 0);
+                    K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, u2, K2[mt], 0, 0, 0);
+                    K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bi, u3, K3[mt], 0, 0, 0);
+                }
             }
-            // weight sum over the 128 elements of voxel i (16 lanes x 8 tiles)
-            asum += __shfl_xor(asum, 1);
-            asum += __shfl_xor(asum, 2);
-            asum += __shfl_xor(asum, 4);
-            asum += __shfl_xor(asum, 8);
-            const float sc = (klo <= khi && a_ok) ? A.out_scale * (A.norm_rx ? 1.0f / asum : 1.0f) : 0.f;
-            // U = s y conj(T) / w
-            const float2 yv = vy[i];
-            const float rt = rev_of(n * q_tx[i * 16 + cidx]);
-            const float st = __builtin_amdgcn_sinf(rt), ct = __builtin_amdgcn_cosf(rt);
-            const float ur = sc * (yv.x * ct + yv.y * st);
-            const float ui = sc * (yv.y * ct - yv.x * st);
-            const float u2 = ui - ur, u3 = ur + ui;
-            const uint4 qa = *reinterpret_cast<const uint4 *>(&q_rx[i * ADJ_QS + cidx * ADJ_ET]);
-            const uint4 qb = *reinterpret_cast<const uint4 *>(&q_rx[i * ADJ_QS + cidx * ADJ_ET + 4]);
-            const uint32_t Q[ADJ_ET] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
-#pragma unroll
-            for (int mt = 0; mt < ADJ_ET; ++mt) {
-                const float rv = rev_of(n * Q[mt]);
-                const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                const float br = wv[mt] * cs, bi = wv[mt] * sn;
-                K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br - bi, ur, K1[mt], 0, 0, 0);
-                K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, u2, K2[mt], 0, 0, 0);
-                K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bi, u3, K3[mt], 0, 0, 0);
-            }
-        }
+        }  // active
+        __syncthreads();  // group `base` consumed, group `base + 16` loaded
     }
     if (!active) return;
     // lane holds D[4q + r][cidx] = Z(element 16 mt + 4q + r, angle column cidx) of the range;
@@ -172,6 +183,37 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void a
             dst[o] = K1[mt][r] - K3[mt][r];
             dst[ADJ_ET * 256 + o] = K1[mt][r] + K2[mt][r];
         }
+}
+
+// Phase words of every voxel for the voxel contraction: FP64 time of flight
+// (das_pw.m:336, 339-340, 427-428 with c_tof and the delay offsets) reduced to
+// frac(f_s tau / N) in units of 2^-32.  qrx[v][cidx][mt] holds element
+// cidx + 16 mt (zero-padded to 128 elements), qtx[v][a] angle a (16 n_at columns).
+__global__ __launch_bounds__(256) void adjoint_tables_kernel(BeamArgs A, uint32_t *__restrict__ qrx,
+                                                             uint32_t *__restrict__ qtx, int n_at)
+{
+    const long n_vox = (long)A.n_x * A.n_z;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const int n_cols = 128 + 16 * n_at;
+    if (i >= n_vox * n_cols) return;
+    const long v = i / n_cols;
+    const int col = (int)(i - v * n_cols);
+    const int ix = (int)(v / A.n_z), iz = (int)(v - (long)ix * A.n_z);
+    const double x = A.px[ix], z = A.pz[iz];
+    if (col < 128) {
+        const int e = col;
+        const double lat = (((double)e - A.M) * A.pitch - x) / A.ctof, zc = z / A.ctof;
+        const double tau = sqrt(lat * lat + zc * zc) + (e < A.n_el ? A.delay[e] : 0.0);
+        qrx[v * 128 + (e & 15) * 8 + (e >> 4)] = frac32(A.f_s * tau / A.N);
+    } else {
+        const int a = col - 128;
+        double tau = 0.0;
+        if (a < A.n_tx) {
+            const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
+            tau = (ex * (x - xr) + ez * z) / A.ctof + A.t0 / A.f_s;
+        }
+        qtx[v * 16 * n_at + a] = frac32(A.f_s * tau / A.N);
+    }
 }
 
 // Z[b][a][k] = sum over voxel ranges of the partials (one thread per output)
@@ -281,6 +323,10 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restr
 
 hipError_t launch_adjoint(int wcr, dim3 grid, hipStream_t s, const AdjArgs &S)
 {
+    const long n_vox = (long)S.B.n_x * S.B.n_z;
+    const long n_tab = n_vox * (128 + 16 * S.n_at);
+    hipLaunchKernelGGL(adjoint_tables_kernel, dim3((unsigned)((n_tab + 255) / 256)), dim3(256), 0, s, S.B,
+                       const_cast<uint32_t *>(S.qrx), const_cast<uint32_t *>(S.qtx), S.n_at);
     const dim3 block(64 * ADJ_WAVES);
     switch (wcr) {
     case WC_RANGE: hipLaunchKernelGGL(adjoint_kernel<WC_RANGE>, grid, block, 0, s, S); break;

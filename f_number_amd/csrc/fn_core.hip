@@ -89,6 +89,7 @@ struct fn_plan {
     // adjoint / vector-Jacobian workspace (allocated on first use)
     float *d_P = nullptr;       // voxel-range partials of the voxel contraction
     float2 *d_Z = nullptr;      // reduced contraction [n_f][n_tx][n_el]
+    uint32_t *d_qtab = nullptr; // per-voxel phase words of the contraction
     double *d_part = nullptr;   // reduction partials
     int adj_vr = 0, adj_at = 0;
     long adj_vpr = 0;
@@ -185,7 +186,7 @@ void destroy(fn_plan *p)
     if (!p) return;
     hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp); hipFree(p->d_sqf);
     hipFree(p->d_tw); hipFree(p->d_X); hipFree(p->d_Xlp); hipFree(p->d_xscale);
-    hipFree(p->d_delay); hipFree(p->d_P); hipFree(p->d_Z); hipFree(p->d_part);
+    hipFree(p->d_delay); hipFree(p->d_P); hipFree(p->d_Z); hipFree(p->d_part); hipFree(p->d_qtab);
     for (auto &r : p->pending) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     for (auto &r : p->spare) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     if (p->stream) hipStreamDestroy(p->stream);
@@ -569,11 +570,15 @@ int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
         HIP_CHECK(hipMalloc(&P->d_P, p_floats * sizeof(float)), "adjoint partials");
         HIP_CHECK(hipMalloc(&P->d_Z, (size_t)n_f * P->n_tx * P->n_el * sizeof(float2)), "adjoint contraction");
         HIP_CHECK(hipMalloc(&P->d_part, 1024 * sizeof(double)), "reduction partials");
+        HIP_CHECK(hipMalloc(&P->d_qtab, (size_t)P->n_vox * (128 + 16 * P->adj_at) * sizeof(uint32_t)),
+                  "phase tables");
     }
     fnk::AdjArgs S{};
     S.B = P->args;
     S.y = reinterpret_cast<const float2 *>(d_y);
     S.P = P->d_P;
+    S.qrx = P->d_qtab;
+    S.qtx = P->d_qtab + (size_t)P->n_vox * 128;
     S.vox_per_range = P->adj_vpr;
     S.n_at = P->adj_at;
     const dim3 grid((unsigned)P->adj_vr, (unsigned)n_chunks, (unsigned)P->adj_at);

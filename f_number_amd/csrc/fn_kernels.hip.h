@@ -86,6 +86,8 @@ struct AdjArgs {
     BeamArgs B;
     const float2 *y;        // cotangent image [n_vox]
     float *P;               // partials [n_vr][n_f][n_at][2][128][16]
+    const uint32_t *qrx;    // receive phase words [n_vox][cidx 16][mt 8] (element cidx + 16 mt)
+    const uint32_t *qtx;    // transmit phase words [n_vox][16 n_at]
     long vox_per_range;     // multiple of 16
     int n_at;               // angle tiles of 16
 };

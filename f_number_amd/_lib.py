@@ -47,7 +47,7 @@ class PwParams(C.Structure):
         ("f_number_values", _dp), ("n_f", C.c_int32),
         ("normalization", C.c_int32), ("normalization_window", Window),
         ("semantics", C.c_int32), ("precision", C.c_int32), ("output_scale", C.c_int32),
-        ("index_gpu", C.c_int32), ("verbosity", C.c_int32),
+        ("index_gpu", C.c_int32), ("verbosity", C.c_int32), ("n_frames", C.c_int32),
     ]
 
 

@@ -80,6 +80,45 @@ hipError_t FN_TAN_LAUNCHER(INST_KI)(int mt, int wcr, dim3 grid, size_t lds, hipS
 }
 #endif
 
+#if INST_MODE == 0
+// ensemble variant: frames x angles as transmit rows, up to four tiles per group
+#define FN_ENS_LAUNCHER(k, p) FN_CAT6(launch_beamform_ens, _k, k, _p, p, )
+template <int MT, int WCR>
+static hipError_t launch_ens_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    auto kern = beamform_kernel<0, MT, INST_KI, WCR, WC_RANGE, INST_PREC, false, true>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
+    return hipGetLastError();
+}
+
+template <int MT>
+static hipError_t launch_ens_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    switch (wcr) {
+    case WC_RANGE: return launch_ens_one<MT, WC_RANGE>(grid, lds, s, A);
+    case WC_LINEAR: return launch_ens_one<MT, WC_LINEAR>(grid, lds, s, A);
+    default: return launch_ens_one<MT, WC_COS>(grid, lds, s, A);
+    }
+}
+
+hipError_t FN_ENS_LAUNCHER(INST_KI, INST_PREC)(int mt, int wcr, dim3 grid, size_t lds, hipStream_t s,
+                                                const BeamArgs &A)
+{
+    switch (mt) {
+    case 1: return launch_ens_mt<1>(wcr, grid, lds, s, A);
+    case 2: return launch_ens_mt<2>(wcr, grid, lds, s, A);
+    case 3: return launch_ens_mt<3>(wcr, grid, lds, s, A);
+    case 4: return launch_ens_mt<4>(wcr, grid, lds, s, A);
+    default: return hipErrorInvalidValue;
+    }
+}
+#endif
+
 hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
                                             const BeamArgs &A)
 {

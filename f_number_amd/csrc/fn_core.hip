@@ -27,6 +27,21 @@ FN_DECL_LAUNCHERS(1, 8)
 FN_DECL_LAUNCHERS(1, 16)
 FN_DECL_LAUNCHERS(1, 32)
 #undef FN_DECL_LAUNCHERS
+#define FN_DECL_ENS(k)                                                                              \
+    hipError_t launch_beamform_ens_k##k##_p0(int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
+    hipError_t launch_beamform_ens_k##k##_p1(int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
+    hipError_t launch_beamform_ens_k##k##_p2(int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
+    hipError_t launch_beamform_ens_k##k##_p3(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+FN_DECL_ENS(8)
+FN_DECL_ENS(16)
+FN_DECL_ENS(32)
+#undef FN_DECL_ENS
+typedef hipError_t (*EnsLauncher)(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+// [KI = 8, 16, 32][precision]
+static const EnsLauncher kEnsLaunchers[3][4] = {
+    {launch_beamform_ens_k8_p0, launch_beamform_ens_k8_p1, launch_beamform_ens_k8_p2, launch_beamform_ens_k8_p3},
+    {launch_beamform_ens_k16_p0, launch_beamform_ens_k16_p1, launch_beamform_ens_k16_p2, launch_beamform_ens_k16_p3},
+    {launch_beamform_ens_k32_p0, launch_beamform_ens_k32_p1, launch_beamform_ens_k32_p2, launch_beamform_ens_k32_p3}};
 hipError_t launch_beamform_tan_k8(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k16(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k32(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
@@ -75,6 +90,7 @@ struct fn_plan {
     int n_x = 0, n_z = 0, n_t = 0, n_el = 0, n_tx = 0;
     long n_vox = 0;
     int KI = 0, MT_all = 0, MT_tmpl = 0, n_groups = 1, bin_split = 1;
+    int n_frames = 1;  // ensemble frames (transmit rows = frames x angles)
     int wcr = 0, wct = 0;
     int precision = FN_PRECISION_FP32;
     int verbosity = 0;
@@ -219,7 +235,9 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     P->MT_all = (c.n_tx + 15) / 16;
     if (P->mode == 0) {
         // at most 5 transmit tiles per group keep the LDS tile within 80 KiB
-        const int groups = (P->MT_all + 4) / 5;
+        // (ensembles: 4, for the per-row image accumulators)
+        const int max_tiles = P->n_frames > 1 ? 4 : 5;
+        const int groups = (P->MT_all + max_tiles - 1) / max_tiles;
         P->MT_tmpl = (P->MT_all + groups - 1) / groups;
         P->n_groups = (P->MT_all + P->MT_tmpl - 1) / P->MT_tmpl;
     } else {
@@ -322,6 +340,7 @@ double useful_fraction(const fnh::Grid &g, const double *px, int nx, const doubl
 hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipStream_t s)
 {
     const int kidx = P->KI == 8 ? 0 : (P->KI == 16 ? 1 : 2);
+    if (P->n_frames > 1) return fnk::kEnsLaunchers[kidx][P->precision](P->MT_tmpl, P->wcr, grid, P->lds_beam, s, A);
     return fnk::kBeamLaunchers[P->mode][kidx][P->precision](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
 }
 
@@ -373,8 +392,8 @@ int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w,
             HIP_CHECK(hipEventCreate(&run.e2), "hipEventCreate");
         }
     }
-    if (A.atomic_out) {
-        const long n = 2 * P->n_vox;
+    if (A.atomic_out || P->n_frames > 1) {
+        const long n = 2 * P->n_vox * P->n_frames;
         hipLaunchKernelGGL(fnk::fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_image, n);
         HIP_CHECK(hipGetLastError(), "fill_zero_kernel");
     }
@@ -433,25 +452,29 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
     fn_plan *P = new (std::nothrow) fn_plan();
     if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
     P->mode = 0;
+    P->n_frames = std::max(1, p->n_frames);
     int wsup = 0;
     if ((s = setup_window(p->window, p->semantics, &P->args.wrx, &P->wcr, &wsup))) { destroy(P); return s; }
     P->args.support_rx = wsup;
     P->args.norm_rx = p->normalization != FN_NORMALIZATION_OFF;
     P->args.clamp = p->semantics == FN_SEMANTICS_CPU;
-    Common c{p->positions_x, p->positions_z, p->n_x, p->n_z, p->n_samples, p->n_elements, p->n_angles,
+    // transmit rows: the angles of every frame (row = frame * n_angles + angle)
+    const int n_rows = p->n_angles * P->n_frames;
+    P->args.rows_per_frame = P->n_frames > 1 ? p->n_angles : 0;
+    Common c{p->positions_x, p->positions_z, p->n_x, p->n_z, p->n_samples, p->n_elements, n_rows,
              p->f_s, p->element_width, p->element_pitch, p->c_0, p->index_t0,
              p->semantics, p->precision, p->output_scale, p->index_gpu, p->verbosity};
     if ((s = create_common(P, c, g, p->f_number_values, nullptr))) { destroy(P); return s; }
     // steering geometry in FP64: das_pw.m:323-327 (x_ref = sign(e_x) x_0)
-    std::vector<double> txp(3 * (size_t)p->n_angles);
+    std::vector<double> txp(3 * (size_t)n_rows);
     const double M = (p->n_elements - 1) / 2.0;
-    for (int a = 0; a < p->n_angles; ++a) {
-        const double th = p->steering_angles[a];
+    for (int row = 0; row < n_rows; ++row) {
+        const double th = p->steering_angles[row % p->n_angles];
         const double ex = -std::sin(th), ez = std::cos(th);
         const double sg = (ex > 0) - (ex < 0);
-        txp[3 * a] = ex;
-        txp[3 * a + 1] = ez;
-        txp[3 * a + 2] = sg * ((0 - M) * p->element_pitch);
+        txp[3 * row] = ex;
+        txp[3 * row + 1] = ez;
+        txp[3 * row + 2] = sg * ((0 - M) * p->element_pitch);
     }
     if ((s = upload(&P->d_txp, txp.data(), txp.size(), "steering"))) { destroy(P); return s; }
     P->args.txp = P->d_txp;
@@ -515,6 +538,7 @@ int check_pw_fp32(const fn_plan *P, const char *what)
 {
     if (!P) return fail(FN_ERR_ARGUMENT, "null plan");
     if (P->mode != 0) return fail(FN_ERR_UNSUPPORTED, "%s: plane-wave plans only", what);
+    if (P->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "%s: single-frame plans only", what);
     if (P->n_el > 128) return fail(FN_ERR_UNSUPPORTED, "%s: at most 128 elements", what);
     return FN_OK;
 }
@@ -629,7 +653,7 @@ int vjp(fn_plan *P, const void *d_rf, int dtype, const float *d_g, const float *
 int one_shot(fn_plan *P, const void *rf, int dtype, float *image, float *w0, float *w1)
 {
     const size_t rf_bytes = (size_t)P->n_t * P->n_el * P->n_tx * dtype_size(dtype);
-    const size_t img_bytes = (size_t)P->n_vox * 2 * sizeof(float);
+    const size_t img_bytes = (size_t)P->n_vox * 2 * sizeof(float) * P->n_frames;
     const size_t w_bytes = (size_t)P->n_vox * sizeof(float);
     void *d_rf = nullptr;
     float *d_img = nullptr, *d_w0 = nullptr, *d_w1 = nullptr;

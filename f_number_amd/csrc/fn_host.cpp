@@ -294,6 +294,8 @@ int validate_pw(const fn_pw_params *p, Grid *g)
                          p->precision, p->output_scale, p->index_gpu, p->verbosity);
     if (s) return s;
     if (p->n_angles < 1 || !p->steering_angles) return fail(FN_ERR_STEERING_ANGLE, "Steering angle missing");
+    if (p->n_frames < 0 || (long)std::max(1, p->n_frames) * p->n_angles > (1L << 24))
+        return fail(FN_ERR_ARGUMENT, "n_frames must be in 0..2^24 / n_angles (got %d)", p->n_frames);
     for (int a = 0; a < p->n_angles; ++a) {
         const double th = p->steering_angles[a];
         if (!(th > -M_PI / 2 && th < M_PI / 2))

@@ -74,6 +74,7 @@ struct BeamArgs {
     int norm_rx, norm_tx;
     int atomic_out;
     double M, pitch, width, c0, f_s, t0;
+    int rows_per_frame;     // ensemble: transmit rows per frame (angles); images [frame][n_vox]
     double ctof;            // speed of sound of the time of flight (c0 unless set)
     const double *delay;    // per-element delay offsets (s), [n_el]
     float *dimage;          // tangent: dI / dc_tof, interleaved complex [n_vox]
@@ -329,11 +330,15 @@ struct WavesPerEU {
 // TAN (plane waves, FP32): also accumulates the tangent dI/dc_tof.  Geometric
 // delays scale as 1/c, so d/dc e^{j w tau} = j w (-(tau_tx,g + tau_rx,g) / c) e^{j w tau}:
 // a second contraction with B' = B tau_rx,g and the row factor tau_tx,g.
-template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false>
+// ENS (plane waves): the transmit rows are (frame, angle) pairs of an ensemble
+// sharing one geometry; every row keeps its own image accumulator over the
+// bins and is added to its frame's image at the end (frame = row / rows_per_frame).
+template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<MT, KI>::value)))
 void beamform_kernel(BeamArgs A)
 {
     static_assert(!TAN || (MODE == MODE_PW && PREC == PREC_FP32), "tangent: plane waves, FP32");
+    static_assert(!ENS || (MODE == MODE_PW && !TAN), "ensemble: plane waves");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c = lane & 15;
@@ -393,6 +398,13 @@ void beamform_kernel(BeamArgs A)
 
     float Ir = 0.f, Ii = 0.f, w_last = 0.f, wtx_last = 0.f;
     float Dr = 0.f, Di = 0.f;  // tangent accumulators
+    float Er[ENS ? MT : 1][4], Ei[ENS ? MT : 1][4];  // ensemble: per-row image accumulators
+    if (ENS) {
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Er[ENS ? mt : 0][r] = Ei[ENS ? mt : 0][r] = 0.f;
+    }
     // w_n = n * 2 pi f_s / N; the tangent factor -w_n / c_tof is applied per bin
     const float w_unit = (float)(2.0 * 3.141592653589793 * A.f_s / A.N);
     const float inv_c = (float)(1.0 / A.ctof);
@@ -566,8 +578,13 @@ void beamform_kernel(BeamArgs A)
                         const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
                         const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
                         const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
-                        Ir += cs * yr - sn * yi;
-                        Ii += cs * yi + sn * yr;
+                        if (ENS) {
+                            Er[ENS ? mt : 0][r] += cs * yr - sn * yi;
+                            Ei[ENS ? mt : 0][r] += cs * yi + sn * yr;
+                        } else {
+                            Ir += cs * yr - sn * yi;
+                            Ii += cs * yi + sn * yr;
+                        }
                         if (TAN) {
                             const float tt = Ttx[TAN ? mt : 0][r];
                             const float ur = fmaf(tt, accR[mt][r], tanR[TAN ? mt : 0][r]);
@@ -622,6 +639,37 @@ void beamform_kernel(BeamArgs A)
             if (b == A.n_f - 1) wtx_last = tsum;
         }
         if (b == A.n_f - 1) w_last = asum;
+    }
+    if (ENS) {
+        // rows of this lane -> their frames; consecutive rows of one frame are summed first
+        if (vok) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                float sr = 0.f, si = 0.f;
+                int f_run = -1;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * (mt0 + mt) + 4 * g + r;
+                    const int f = row < A.n_tx ? row / A.rows_per_frame : -1;
+                    if (f != f_run) {
+                        if (f_run >= 0) {
+                            atomicAdd(&A.image[(size_t)f_run * n_vox + v].x, A.out_scale * sr);
+                            atomicAdd(&A.image[(size_t)f_run * n_vox + v].y, A.out_scale * si);
+                        }
+                        f_run = f;
+                        sr = si = 0.f;
+                    }
+                    sr += Er[ENS ? mt : 0][r];
+                    si += Ei[ENS ? mt : 0][r];
+                }
+                if (f_run >= 0) {
+                    atomicAdd(&A.image[(size_t)f_run * n_vox + v].x, A.out_scale * sr);
+                    atomicAdd(&A.image[(size_t)f_run * n_vox + v].y, A.out_scale * si);
+                }
+            }
+            if (g == 0 && b_end == A.n_f && blockIdx.z == 0 && A.weights) A.weights[v] = w_last;
+        }
+        return;
     }
     Ir += __shfl_xor(Ir, 16);
     Ir += __shfl_xor(Ir, 32);

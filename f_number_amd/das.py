@@ -122,7 +122,7 @@ def _image_out(buf, n_x, n_z):
 
 
 def pw_params(px, pz, n_t, n_el, angles, f_s, w, p, c, fb, t0, window, F, normalization, index_gpu=0,
-              semantics="cpu", precision="fp32", output_scale="mex", verbosity=0):
+              semantics="cpu", precision="fp32", output_scale="mex", verbosity=0, n_frames=1):
     """Build the fn_pw_params struct (and the arrays it points to) for libfnumber_hip."""
     L = _lib.load()
     px = np.ascontiguousarray(px, dtype=np.float64)
@@ -145,6 +145,7 @@ def pw_params(px, pz, n_t, n_el, angles, f_s, w, p, c, fb, t0, window, F, normal
     P.output_scale = _lib.SCALE[output_scale]
     P.index_gpu = int(index_gpu)
     P.verbosity = int(verbosity)
+    P.n_frames = int(n_frames)
     N, lb, ub = C.c_int32(), C.c_int32(), C.c_int32()
     _lib.check(L.fn_pw_frequency_grid(C.byref(P), C.byref(N), C.byref(lb), C.byref(ub)), "gpu_bf_das_pw_rf")
     r = np.empty(ub.value - lb.value + 1)

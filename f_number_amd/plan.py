@@ -48,7 +48,7 @@ class _Plan:
         if image.dtype != torch.complex64:
             raise _lib.FNumberError(f"{self.prefix}:InvalidArgument", "image must be complex64")
         inf = self.info()
-        if image.numel() != inf["n_voxels"]:
+        if image.numel() != inf["n_voxels"] * getattr(self, "n_frames", 1):
             raise _lib.FNumberError(f"{self.prefix}:InvalidArgument", "image size does not match the grid")
         if rf.numel() != self._rf_numel:
             raise _lib.FNumberError(f"{self.prefix}:InvalidRFData", "RF size does not match the plan")
@@ -120,20 +120,23 @@ class PwPlan(_Plan):
 
     def __init__(self, positions_x, positions_z, n_samples, n_elements, steering_angles, f_s, element_width,
                  element_pitch, c_0, f_bounds, index_t0=0.0, window=None, F_number=None, normalization=None,
-                 index_gpu=0, semantics="cpu", precision="fp32", output_scale="mex"):
+                 index_gpu=0, semantics="cpu", precision="fp32", output_scale="mex", n_frames=1):
+        """n_frames > 1: an ensemble of frames with this geometry -- RF (n_frames, N_a, N_el, N_t),
+        images (n_frames, N_x, N_z), each frame compounding its own angles."""
         super().__init__()
         window = window if window is not None else windows.tukey(0.2)
         F_number = F_number if F_number is not None else f_numbers.grating.angle_lb(60, 1.5)
         normalization = normalization if normalization is not None else normalizations.on()
         P, keep, info = pw_params(positions_x, positions_z, n_samples, n_elements, steering_angles, f_s,
                                   element_width, element_pitch, c_0, f_bounds, index_t0, window, F_number,
-                                  normalization, index_gpu, semantics, precision, output_scale)
+                                  normalization, index_gpu, semantics, precision, output_scale, n_frames=n_frames)
         _lib.check(_lib.load().fn_plan_pw_create(C.byref(P), C.byref(self._h)), self.prefix)
         self._keep = keep
         self.grid = info
         self.n_samples, self.n_elements, self.n_angles = int(n_samples), int(n_elements), len(keep[2])
+        self.n_frames = max(1, int(n_frames))
         self.c_0 = float(c_0)
-        self._rf_numel = self.n_samples * self.n_elements * self.n_angles
+        self._rf_numel = self.n_samples * self.n_elements * self.n_angles * self.n_frames
 
     def set_tof(self, c_tof=0.0, delay_offsets=None):
         """Speed of sound of the time of flight (0: the plan's c_0) and per-element

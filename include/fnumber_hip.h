@@ -135,6 +135,11 @@ typedef struct fn_pw_params {
     int32_t output_scale;            /* FN_SCALE_* */
     int32_t index_gpu;
     int32_t verbosity;               /* 0..3 (gpu_bf_das_pw_rf.cu:469-485) */
+    /* ensemble (no counterpart in the reference): n_frames acquisitions of the
+     * same geometry beamformed in one pass, each frame compounding its
+     * n_angles; RF [n_frames][n_angles][n_elements][n_samples], images
+     * [n_frames][n_x][n_z].  0 or 1: one frame (the reference call). */
+    int32_t n_frames;
 } fn_pw_params;
 
 /* Full synthetic aperture (das_sa.m:1, gpu_bf_saft_rf.cu:107).

@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--precisions", default="bf16x3,fp16x3,fp16",
                     help="mixed-precision arms of the headline configuration (comma list, '' for none)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form measurements")
+    ap.add_argument("--no-batching", action="store_true", help="skip the batching-form measurements")
+    ap.add_argument("--ensemble-frames", type=int, default=400)
+    ap.add_argument("--queue-jobs", type=int, default=64)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
@@ -174,6 +177,112 @@ def run_diff(steps, warmup, dev):
     }
 
 
+def run_batching(steps, n_frames, n_jobs, dev):
+    """Batching form: a Doppler ensemble (n_frames x 5 angles, 1536 samples, 256 x 256,
+    flow phantom) and a queue of n_jobs independent CPWC jobs (grids and angle counts drawn
+    with a fixed seed) executed together as one captured graph over 4 streams, every plan
+    autotuned; mixed precision (fp16) and the same batch in FP32."""
+    import numpy as np
+    import torch
+
+    import f_number_amd as fa
+    from f_number_amd import synth
+
+    from f_number_amd.tuning import TilingCache
+
+    t_setup = time.perf_counter()
+    w0 = synth.doppler_frame(0)
+    n_vox_e = w0.n_voxels
+    rf_e = torch.empty((n_frames, w0.n_tx, w0.n_elements, w0.n_samples), dtype=torch.float32, device=dev)
+    for f in range(n_frames):
+        wf = synth.doppler_frame(f)
+        wf.n_speckle = 5000  # synthesis time only; the beamforming cost does not depend on the scene
+        rf_e[f] = wf.rf_device(dev)
+    rng = np.random.default_rng(20260003 + 3000)
+    grids = [(128, 128), (256, 384), (512, 512), (512, 768)]
+    jobs = []
+    for j in range(n_jobs):
+        nz, nx = grids[rng.integers(len(grids))]
+        na = int(rng.choice([1, 3, 11, 31]))
+        w = synth.Workload(f"job{j}", "pw", 1536, np.linspace(-15.0, 15.0, na) if na > 1 else np.zeros(1),
+                           np.linspace(-19e-3, 19e-3, nx), np.linspace(5e-3, 40e-3, nz), (2.25e6, 6.75e6),
+                           n_speckle=5000, seed=20260003 + 4000 + j)
+        jobs.append((w, w.rf_device(dev)))
+    torch.cuda.synchronize(dev)
+    t_synth = time.perf_counter() - t_setup
+    kw = dict(window=fa.windows.hann(), F_number=fa.f_numbers.grating.angle_lb(60, 1.5))
+
+    cache = TilingCache()
+
+    def build(precision):
+        plans, rfs, outs = [], [], []
+        p = fa.PwPlan(w0.positions_x, w0.positions_z, w0.n_samples, w0.n_elements, w0.angles, w0.f_s, w0.width,
+                      w0.pitch, w0.c_0, w0.f_bounds, 0.0, precision=precision, n_frames=n_frames, **kw)
+        plans.append(p)
+        rfs.append(rf_e)
+        outs.append(torch.empty((n_frames, w0.positions_x.size, w0.positions_z.size), dtype=torch.complex64,
+                                device=dev))
+        for w, rf in jobs:
+            plans.append(fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width,
+                                   w.pitch, w.c_0, w.f_bounds, 0.0, precision=precision, **kw))
+            rfs.append(rf)
+            outs.append(torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device=dev))
+        t0 = time.perf_counter()
+        tunes = [cache.tune(pl, rf, o, repetitions=1) for pl, rf, o in zip(plans, rfs, outs)]
+        torch.cuda.synchronize(dev)
+        return plans, rfs, outs, tunes, time.perf_counter() - t0
+
+    res = {}
+    images = {}
+    for precision in ("fp16", "fp32"):
+        plans, rfs, outs, tunes, t_tune = build(precision)
+        batch = fa.Batch(plans, n_streams=4)
+        ms = _time_ms(lambda: batch.execute(rfs, outs), steps, 1, dev)
+        ens_ms = _time_ms(lambda: plans[0].execute(rfs[0], outs[0]), max(2, steps // 2), 1, dev)
+        images[precision] = [o.clone() for o in outs]
+        info = batch.info()
+        voxel_frames = n_frames * n_vox_e + sum(w.n_voxels for w, _ in jobs)
+        res[precision] = {"batch_ms": ms, "value": voxel_frames / (ms / 1e3) / 1e6, "unit": "Mvoxel*frames/s",
+                          "ensemble_alone_ms": ens_ms, "graph_nodes": info["graph_nodes"],
+                          "autotune_s": t_tune,
+                          "tiling_ensemble": {"m_tiles": tunes[0]["m_tiles"], "bin_split": tunes[0]["bin_split"],
+                                              "default_ms": tunes[0]["default_ms"], "best_ms": tunes[0]["best_ms"]},
+                          "autotune_gain": sum(t["default_ms"] for t in tunes) / sum(t["best_ms"] for t in tunes)}
+        if precision == "fp32":
+            # the plain FP32 path: every frame and every job as its own plan execution on one
+            # stream (default tilings, no frame batching, no graph)
+            single = fa.PwPlan(w0.positions_x, w0.positions_z, w0.n_samples, w0.n_elements, w0.angles, w0.f_s,
+                               w0.width, w0.pitch, w0.c_0, w0.f_bounds, 0.0, **kw)
+            one = torch.empty((w0.positions_x.size, w0.positions_z.size), dtype=torch.complex64, device=dev)
+            per = _time_ms(lambda: [single.execute(rf_e[f], one) for f in range(n_frames)], 2, 1, dev)
+            res["ensemble_frames_as_rows_speedup_fp32"] = per / ens_ms
+            single.close()
+            plain = [fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width,
+                               w.pitch, w.c_0, w.f_bounds, 0.0, **kw) for w, _ in jobs]
+            jobs_ms = _time_ms(lambda: [pl.execute(rf, o) for pl, (_, rf), o in zip(plain, jobs, outs[1:])], 2, 1,
+                               dev)
+            for pl in plain:
+                pl.close()
+            res["plain_fp32_ms"] = per + jobs_ms
+        batch.close()
+        for pl in plans:
+            pl.close()
+    num = sum(torch.linalg.vector_norm(a - b).item() ** 2 for a, b in zip(images["fp16"], images["fp32"]))
+    den = sum(torch.linalg.vector_norm(b).item() ** 2 for b in images["fp32"])
+    return {"ensemble": {"frames": n_frames, "angles": int(w0.n_tx), "grid": [int(w0.positions_z.size),
+                                                                              int(w0.positions_x.size)],
+                         "n_samples": w0.n_samples},
+            "queue_jobs": n_jobs, "voxel_frames_per_batch": int(n_frames * n_vox_e + sum(w.n_voxels for w, _ in jobs)),
+            "mixed_fp16": res["fp16"], "fp32": res["fp32"],
+            "mixed_vs_fp32_throughput": res["fp16"]["value"] / res["fp32"]["value"],
+            "plain_fp32": {"ms": res["plain_fp32_ms"],
+                           "value": (n_frames * n_vox_e + sum(w.n_voxels for w, _ in jobs)) / (res["plain_fp32_ms"] / 1e3) / 1e6,
+                           "how": "every frame and job as its own FP32 plan execution, default tiling, one stream"},
+            "mixed_batched_vs_plain_fp32": res["plain_fp32_ms"] / res["fp16"]["batch_ms"],
+            "ensemble_frames_as_rows_speedup_fp32": res["ensemble_frames_as_rows_speedup_fp32"],
+            "mixed_rel_l2_vs_fp32": (num / den) ** 0.5, "synthesis_s": t_synth}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -261,6 +370,8 @@ def main():
     }
     if not args.no_diff:
         out["differentiation"] = run_diff(max(3, args.steps // 2), 1, dev)
+    if not args.no_batching:
+        out["batching"] = run_batching(max(2, args.steps // 3), args.ensemble_frames, args.queue_jobs, dev)
     if not args.no_cpu_baseline:
         nthreads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(w, "hann", "angle_lb", (60, 1.5, 10), nthreads)

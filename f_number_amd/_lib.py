@@ -93,6 +93,11 @@ class SynthParams(C.Structure):
     ]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("best_ms", C.c_double), ("default_ms", C.c_double), ("m_tiles", C.c_int32),
+                ("bin_split", C.c_int32), ("n_candidates", C.c_int32)]
+
+
 # every symbol include/fnumber_hip.h declares, with its prototype
 PROTOTYPES = {
     "fn_abi_version": (C.c_int32, []),
@@ -117,6 +122,14 @@ PROTOTYPES = {
     "fn_plan_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
     "fn_plan_get_profile": (C.c_int32, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32)]),
     "fn_plan_destroy": (C.c_int32, [C.c_void_p]),
+    "fn_plan_autotune": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                     C.POINTER(TuneResult), C.c_void_p]),
+    "fn_plan_set_tiling": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
+    "fn_batch_create": (C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "fn_batch_execute": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_void_p), C.c_void_p]),
+    "fn_batch_info": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "fn_batch_destroy": (C.c_int32, [C.c_void_p]),
     "fn_plan_set_tof": (C.c_int32, [C.c_void_p, C.c_double, _dp]),
     "fn_plan_execute_tangent": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),

@@ -349,10 +349,12 @@ hipError_t launch_adjoint_idft(const float2 *Z, const float2 *tw, float *out, in
                                int n_f, hipStream_t s)
 {
     const size_t lds = (size_t)N * sizeof(float2);
-    if (lds > 65536) {
+    static size_t lds_done = 0;  // raised once per process and size (graph capture issues no attribute calls)
+    if (lds > 65536 && lds > lds_done) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(adjoint_idft_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
+        lds_done = lds;
     }
     hipLaunchKernelGGL(adjoint_idft_kernel, dim3((unsigned)((n_t + 255) / 256), (unsigned)n_ch), dim3(256), lds, s, Z,
                        tw, out, n_t, n_ch, N, n_lb, n_f);

@@ -18,15 +18,25 @@
 
 namespace fnk {
 
+// Raise the dynamic-LDS limit of a kernel once per process and size (`done` is a
+// static of the calling launcher, one per kernel instantiation), so that a launch
+// sequence captured into a graph issues no attribute calls.
+template <typename K>
+static hipError_t set_lds_once(K kern, size_t lds, size_t &done)
+{
+    if (lds <= 65536 || lds <= done) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e == hipSuccess) done = lds;
+    return e;
+}
+
 template <int MT, int WCR, int WCT>
 static hipError_t launch_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
     auto kern = beamform_kernel<INST_MODE, MT, INST_KI, WCR, WCT, INST_PREC>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
+    static size_t lds_done = 0;
+    if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
     hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
     return hipGetLastError();
 }
@@ -39,6 +49,7 @@ static hipError_t launch_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, const
     switch (wcr) {
     case WC_RANGE: return launch_one<MT, WC_RANGE, WC_RANGE>(grid, lds, s, A);
     case WC_LINEAR: return launch_one<MT, WC_LINEAR, WC_RANGE>(grid, lds, s, A);
+    case WC_COS_ALL: return launch_one<MT, WC_COS_ALL, WC_RANGE>(grid, lds, s, A);
     default: return launch_one<MT, WC_COS, WC_RANGE>(grid, lds, s, A);
     }
 }
@@ -51,11 +62,8 @@ template <int MT, int WCR>
 static hipError_t launch_tan_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
     auto kern = beamform_kernel<0, MT, INST_KI, WCR, WC_RANGE, 0, true>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
+    static size_t lds_done = 0;
+    if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
     hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
     return hipGetLastError();
 }
@@ -87,11 +95,8 @@ template <int MT, int WCR>
 static hipError_t launch_ens_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
     auto kern = beamform_kernel<0, MT, INST_KI, WCR, WC_RANGE, INST_PREC, false, true>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
+    static size_t lds_done = 0;
+    if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
     hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
     return hipGetLastError();
 }
@@ -102,6 +107,7 @@ static hipError_t launch_ens_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, c
     switch (wcr) {
     case WC_RANGE: return launch_ens_one<MT, WC_RANGE>(grid, lds, s, A);
     case WC_LINEAR: return launch_ens_one<MT, WC_LINEAR>(grid, lds, s, A);
+    case WC_COS_ALL: return launch_ens_one<MT, WC_COS_ALL>(grid, lds, s, A);
     default: return launch_ens_one<MT, WC_COS>(grid, lds, s, A);
     }
 }
@@ -155,6 +161,7 @@ hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, 
     switch (wcr) {
     case WC_RANGE: return launch_wct<WC_RANGE>(wct, grid, lds, s, A);
     case WC_LINEAR: return launch_wct<WC_LINEAR>(wct, grid, lds, s, A);
+    case WC_COS_ALL: return launch_wct<WC_COS_ALL>(wct, grid, lds, s, A);
     default: return launch_wct<WC_COS>(wct, grid, lds, s, A);
     }
 }

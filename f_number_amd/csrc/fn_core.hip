@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -182,6 +183,18 @@ int setup_window(const fn_window &w, int semantics, fnk::WinCoef *c, int *wc, in
     }
 }
 
+// the raised cosine over every element of a full array (CPU Hann) takes the
+// short element loop (WC_COS_ALL); tangent and adjoint use the general form
+void use_cos_all(fn_plan *P)
+{
+    const fnk::WinCoef &w = P->args.wrx;
+    if (P->wcr == fnk::WC_COS && P->args.support_rx == fnk::SUPPORT_ALL && P->n_el == 4 * P->KI && w.t_off == 0.f &&
+        w.t_flat < 0.f && std::isinf(w.t_cut))
+        P->wcr = fnk::WC_COS_ALL;
+}
+
+int wcr_general(int wcr) { return wcr == fnk::WC_COS_ALL ? fnk::WC_COS : wcr; }
+
 int ki_of(int n_el)
 {
     if (n_el <= 32) return 8;
@@ -344,10 +357,25 @@ hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipS
     return fnk::kBeamLaunchers[P->mode][kidx][P->precision](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
 }
 
+// raise a kernel's dynamic-LDS limit once per process and size (no attribute
+// calls inside captured launch sequences)
+hipError_t lds_limit(const void *kern, size_t lds, size_t &done)
+{
+    if (lds <= 65536 || lds <= done) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) done = lds;
+    return e;
+}
+
 int launch_spectra(fn_plan *P, const void *d_rf, int dtype, hipStream_t s)
 {
     const fnk::SpecArgs &S = P->sargs;
     dim3 sg((unsigned)((S.n_f + 63) / 64), (unsigned)((S.n_ch + 63) / 64));
+    static size_t done_f = 0, done_d = 0;
+    HIP_CHECK(dtype == FN_DTYPE_F32
+                  ? lds_limit(reinterpret_cast<const void *>(fnk::spectra_kernel<float>), P->lds_spec, done_f)
+                  : lds_limit(reinterpret_cast<const void *>(fnk::spectra_kernel<double>), P->lds_spec, done_d),
+              "spectra LDS limit");
     if (dtype == FN_DTYPE_F32) {
         hipLaunchKernelGGL(fnk::spectra_kernel<float>, sg, dim3(256), P->lds_spec, s,
                            reinterpret_cast<const float *>(d_rf), P->d_tw, P->d_X, S);
@@ -478,6 +506,7 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
     }
     if ((s = upload(&P->d_txp, txp.data(), txp.size(), "steering"))) { destroy(P); return s; }
     P->args.txp = P->d_txp;
+    use_cos_all(P);
     P->useful_flops = P->flops * useful_fraction(g, p->positions_x, p->n_x, p->positions_z, p->n_z, p->f_number_values,
                                                  p->f_s, p->c_0, p->element_pitch, p->n_elements, P->args.clamp != 0);
     *out = P;
@@ -509,11 +538,98 @@ int sa_plan(const fn_sa_params *p, fn_plan **out)
     P->useful_flops = P->flops * useful_fraction(g, p->positions_x, p->n_x, p->positions_z, p->n_z,
                                                  p->f_number_rx_values, p->f_s, p->c_0, p->element_pitch,
                                                  p->n_elements, false);
+    use_cos_all(P);
     *out = P;
     return FN_OK;
 }
 
 size_t dtype_size(int dtype) { return dtype == FN_DTYPE_F64 ? 8 : 4; }
+
+// ---- tiling and autotuning ---------------------------------------------
+
+int max_tiles(const fn_plan *P)
+{
+    if (P->mode == 1) return P->KI / 4;
+    return std::min(P->MT_all, P->n_frames > 1 ? 4 : 5);
+}
+
+int set_tiling(fn_plan *P, int mt, int split)
+{
+    if (!P) return fail(FN_ERR_ARGUMENT, "null plan");
+    if (P->mode == 1) {
+        if (mt != P->KI / 4) return fail(FN_ERR_ARGUMENT, "synthetic-aperture plans use %d transmit tiles", P->KI / 4);
+    } else if (mt < 1 || mt > max_tiles(P)) {
+        return fail(FN_ERR_ARGUMENT, "m_tiles must be in 1..%d", max_tiles(P));
+    }
+    const int n_f = P->g.n_f;
+    if (split < 1 || split > n_f) return fail(FN_ERR_ARGUMENT, "bin_split must be in 1..%d", n_f);
+    const int planes = P->precision == FN_PRECISION_FP16 ? 2 : 4;
+    const size_t step_bytes = P->precision == FN_PRECISION_FP32 ? 64 * sizeof(float2) : (size_t)planes * 64 * 2;
+    const size_t lds = (size_t)mt * P->KI * step_bytes;
+    if (lds > kMaxLds) return fail(FN_ERR_ARGUMENT, "transmit tile exceeds LDS");
+    const int chunk = (n_f + split - 1) / split;
+    P->MT_tmpl = mt;
+    P->n_groups = P->mode == 1 ? 1 : (P->MT_all + mt - 1) / mt;
+    P->bin_split = (n_f + chunk - 1) / chunk;
+    P->lds_beam = lds;
+    P->args.tx_group = mt;
+    P->args.bin_chunk = chunk;
+    P->args.atomic_out = (P->bin_split > 1 || P->n_groups > 1) ? 1 : 0;
+    return FN_OK;
+}
+
+int autotune(fn_plan *P, const void *d_rf, int dtype, float *d_image, int reps, fn_tune_result *res, hipStream_t s)
+{
+    if (!P || !d_rf || !d_image || !res) return fail(FN_ERR_ARGUMENT, "null argument");
+    reps = std::max(1, reps);
+    const int mt0 = P->MT_tmpl, split0 = P->bin_split;
+    std::vector<std::pair<int, int>> cand;
+    const int n_f = P->g.n_f;
+    const int mt_lo = P->mode == 1 ? P->KI / 4 : 1, mt_hi = P->mode == 1 ? P->KI / 4 : max_tiles(P);
+    for (int mt = mt_lo; mt <= mt_hi; ++mt) {
+        // distinct group counts only: the tile count of a balanced split
+        if (P->mode == 0) {
+            const int groups = (P->MT_all + mt - 1) / mt;
+            if ((P->MT_all + groups - 1) / groups != mt) continue;
+        }
+        for (int split : {1, 2, 4, 8, 16, split0})
+            if (split <= n_f && std::find(cand.begin(), cand.end(), std::make_pair(mt, split)) == cand.end())
+                cand.emplace_back(mt, split);
+    }
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0), "hipEventCreate");
+    HIP_CHECK(hipEventCreate(&e1), "hipEventCreate");
+    const int prof = P->profiling;
+    P->profiling = 0;
+    double best = 1e300, dflt = -1.0;
+    int best_mt = mt0, best_split = split0, st = FN_OK;
+    for (auto &c : cand) {
+        if ((st = set_tiling(P, c.first, c.second))) break;
+        if ((st = execute(P, d_rf, dtype, d_image, nullptr, nullptr, s))) break;  // warm-up
+        hipEventRecord(e0, s);
+        for (int r = 0; r < reps && !st; ++r) st = execute(P, d_rf, dtype, d_image, nullptr, nullptr, s);
+        hipEventRecord(e1, s);
+        if (st) break;
+        hipError_t e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) { st = hip_fail(e, "autotune timing"); break; }
+        const double per = ms / reps;
+        if (c.first == mt0 && c.second == split0) dflt = per;
+        if (per < best) { best = per; best_mt = c.first; best_split = c.second; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    P->profiling = prof;
+    if (st) { set_tiling(P, mt0, split0); return st; }
+    if ((st = set_tiling(P, best_mt, best_split))) return st;
+    res->best_ms = best;
+    res->default_ms = dflt;
+    res->m_tiles = P->MT_tmpl;
+    res->bin_split = P->bin_split;
+    res->n_candidates = (int32_t)cand.size();
+    return FN_OK;
+}
 
 // ---- differentiation form ----------------------------------------------
 
@@ -568,9 +684,10 @@ int execute_tangent(fn_plan *P, const void *d_rf, int dtype, float *d_image, flo
     }
     const size_t lds = (size_t)mt * P->KI * 64 * sizeof(float2);
     const dim3 grid((unsigned)((P->n_vox + 63) / 64), (unsigned)P->bin_split, (unsigned)groups);
-    hipError_t e = P->KI == 8 ? fnk::launch_beamform_tan_k8(mt, P->wcr, grid, lds, s, A)
-                 : P->KI == 16 ? fnk::launch_beamform_tan_k16(mt, P->wcr, grid, lds, s, A)
-                               : fnk::launch_beamform_tan_k32(mt, P->wcr, grid, lds, s, A);
+    const int wcr = wcr_general(P->wcr);
+    hipError_t e = P->KI == 8 ? fnk::launch_beamform_tan_k8(mt, wcr, grid, lds, s, A)
+                 : P->KI == 16 ? fnk::launch_beamform_tan_k16(mt, wcr, grid, lds, s, A)
+                               : fnk::launch_beamform_tan_k32(mt, wcr, grid, lds, s, A);
     if (e != hipSuccess) return hip_fail(e, "beamform tangent kernel");
     return FN_OK;
 }
@@ -606,7 +723,7 @@ int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
     S.vox_per_range = P->adj_vpr;
     S.n_at = P->adj_at;
     const dim3 grid((unsigned)P->adj_vr, (unsigned)n_chunks, (unsigned)P->adj_at);
-    hipError_t e = fnk::launch_adjoint(P->wcr, grid, s, S);
+    hipError_t e = fnk::launch_adjoint(wcr_general(P->wcr), grid, s, S);
     if (e != hipSuccess) return hip_fail(e, "adjoint_kernel");
     e = fnk::launch_adjoint_reduce(P->d_P, P->d_Z, P->adj_vr, n_f, P->adj_at, P->n_tx, P->n_el, s);
     if (e != hipSuccess) return hip_fail(e, "adjoint_reduce_kernel");
@@ -729,6 +846,19 @@ int32_t fn_plan_execute(fn_plan *P, const void *d_rf, int32_t dtype, float *d_im
     return execute(P, d_rf, dtype, d_image, d_w, d_w_rx, reinterpret_cast<hipStream_t>(stream));
 }
 
+int32_t fn_plan_autotune(fn_plan *P, const void *d_rf, int32_t dtype, float *d_image, int32_t repetitions,
+                         fn_tune_result *result, void *stream)
+{
+    fnh::clear_error();
+    return autotune(P, d_rf, dtype, d_image, repetitions, result, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t fn_plan_set_tiling(fn_plan *P, int32_t m_tiles, int32_t bin_split)
+{
+    fnh::clear_error();
+    return set_tiling(P, m_tiles, bin_split);
+}
+
 int32_t fn_plan_set_tof(fn_plan *P, double c_tof, const double *delay_offsets)
 {
     fnh::clear_error();
@@ -842,6 +972,146 @@ int32_t fn_das_sa(const fn_sa_params *p, const void *data_rf, int32_t rf_dtype, 
     s = one_shot(P, data_rf, rf_dtype, image, weights_tx, weights_rx);
     destroy(P);
     return s;
+}
+
+// ---- batches: many plans captured into one graph ----------------------
+
+struct fn_batch {
+    std::vector<fn_plan *> plans;
+    int device = 0;
+    std::vector<hipStream_t> streams;  // forked worker streams (default priority)
+    hipStream_t capture = nullptr;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void *> rf;
+    std::vector<int32_t> dtypes;
+    std::vector<float *> images;
+    int captures = 0;
+    int64_t n_nodes = 0;
+};
+
+static void batch_release_graph(fn_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int32_t fn_batch_create(fn_plan *const *plans, int32_t n_plans, int32_t n_streams, fn_batch **out)
+{
+    fnh::clear_error();
+    if (!plans || !out || n_plans < 1) return fail(FN_ERR_ARGUMENT, "plans and batch pointer required");
+    *out = nullptr;
+    n_streams = std::max(1, std::min(n_streams, 16));
+    auto *b = new (std::nothrow) fn_batch();
+    if (!b) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
+    b->device = plans[0] ? plans[0]->device : 0;
+    for (int i = 0; i < n_plans; ++i) {
+        if (!plans[i]) { delete b; return fail(FN_ERR_ARGUMENT, "plan %d is null", i); }
+        if (plans[i]->device != b->device) { delete b; return fail(FN_ERR_ARGUMENT, "plans must share one device"); }
+        b->plans.push_back(plans[i]);
+    }
+    int st = FN_OK;
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int i = 0; i < n_streams && e == hipSuccess; ++i) {
+        hipStream_t w = nullptr;
+        hipEvent_t j = nullptr;
+        e = hipStreamCreateWithFlags(&w, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+        if (w) b->streams.push_back(w);
+        if (j) b->joins.push_back(j);
+    }
+    if (e != hipSuccess) st = hip_fail(e, "batch streams");
+    if (st) { fn_batch_destroy(b); return st; }
+    *out = b;
+    return FN_OK;
+}
+
+int32_t fn_batch_execute(fn_batch *b, const void *const *d_rf, const int32_t *dtypes, float *const *d_images,
+                         void *stream)
+{
+    fnh::clear_error();
+    if (!b || !d_rf || !dtypes || !d_images) return fail(FN_ERR_ARGUMENT, "null argument");
+    const size_t n = b->plans.size();
+    for (size_t i = 0; i < n; ++i)
+        if (!d_rf[i] || !d_images[i]) return fail(FN_ERR_ARGUMENT, "buffers of plan %d missing", (int)i);
+    HIP_CHECK(hipSetDevice(b->device), "hipSetDevice");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const bool same = b->exec && b->rf.size() == n &&
+                      std::equal(b->rf.begin(), b->rf.end(), d_rf) &&
+                      std::equal(b->dtypes.begin(), b->dtypes.end(), dtypes) &&
+                      std::equal(b->images.begin(), b->images.end(), d_images);
+    if (!same) {
+        batch_release_graph(b);
+        // eager pass: validates every job and raises kernel LDS limits outside the capture
+        for (size_t i = 0; i < n; ++i) {
+            const int prof = b->plans[i]->profiling;
+            b->plans[i]->profiling = 0;
+            int st = execute(b->plans[i], d_rf[i], dtypes[i], d_images[i], nullptr, nullptr, s);
+            b->plans[i]->profiling = prof;
+            if (st) return st;
+        }
+        HIP_CHECK(hipStreamSynchronize(s), "batch warm-up");
+        HIP_CHECK(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        int st = FN_OK;
+        hipError_t e = hipEventRecord(b->fork, b->capture);
+        for (size_t w = 0; w < b->streams.size() && e == hipSuccess; ++w) e = hipStreamWaitEvent(b->streams[w], b->fork, 0);
+        for (size_t i = 0; i < n && e == hipSuccess && !st; ++i) {
+            const int prof = b->plans[i]->profiling;
+            b->plans[i]->profiling = 0;
+            st = execute(b->plans[i], d_rf[i], dtypes[i], d_images[i], nullptr, nullptr,
+                         b->streams[i % b->streams.size()]);
+            b->plans[i]->profiling = prof;
+        }
+        for (size_t w = 0; w < b->streams.size() && e == hipSuccess && !st; ++w) {
+            e = hipEventRecord(b->joins[w], b->streams[w]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->capture, b->joins[w], 0);
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(b->capture, &g);
+        if (e == hipSuccess) e = e2;
+        if (st) { if (g) hipGraphDestroy(g); return st; }
+        if (e != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(e, "batch capture"); }
+        b->graph = g;
+        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) { batch_release_graph(b); return hip_fail(e, "hipGraphInstantiate"); }
+        size_t nodes = 0;
+        if (hipGraphGetNodes(g, nullptr, &nodes) == hipSuccess) b->n_nodes = (int64_t)nodes;
+        b->rf.assign(d_rf, d_rf + n);
+        b->dtypes.assign(dtypes, dtypes + n);
+        b->images.assign(d_images, d_images + n);
+        b->captures += 1;
+    }
+    HIP_CHECK(hipGraphLaunch(b->exec, s), "hipGraphLaunch");
+    return FN_OK;
+}
+
+int32_t fn_batch_info(const fn_batch *b, int64_t *n_nodes, int32_t *n_captures)
+{
+    if (!b) return fail(FN_ERR_ARGUMENT, "null batch");
+    if (n_nodes) *n_nodes = b->n_nodes;
+    if (n_captures) *n_captures = b->captures;
+    return FN_OK;
+}
+
+int32_t fn_batch_destroy(fn_batch *b)
+{
+    if (!b) return FN_OK;
+    hipSetDevice(b->device);
+    for (hipStream_t w : b->streams) hipStreamSynchronize(w);
+    if (b->capture) hipStreamSynchronize(b->capture);
+    batch_release_graph(b);
+    for (hipStream_t w : b->streams) hipStreamDestroy(w);
+    for (hipEvent_t j : b->joins) hipEventDestroy(j);
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->capture) hipStreamDestroy(b->capture);
+    delete b;
+    return FN_OK;
 }
 
 }  // extern "C"

@@ -35,7 +35,11 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // window classes for the element loop (the generic window below covers the
 // rest with runtime coefficients)
-enum { WC_RANGE = 0, WC_LINEAR = 1, WC_COS = 2 };
+// WC_COS_ALL: the raised cosine A + B cos(pi t) over every element with no flat
+// part and no cut (CPU-semantics Hann, das_pw.m:449-452 with hann.m:62) on a
+// full array (n_el = 4 KI): no support test, the 1/2 of the argument folded
+// into the per-bin reciprocals.
+enum { WC_RANGE = 0, WC_LINEAR = 1, WC_COS = 2, WC_COS_ALL = 3 };
 enum { MODE_PW = 0, MODE_SA = 1 };
 
 // Generic |t| window: 1 for t <= t_flat; for t_flat < t < t_cut:
@@ -470,11 +474,19 @@ void beamform_kernel(BeamArgs A)
             }
             const int klo = ap.klo - g, khi = ap.khi - g;  // support in units of 4j
             const float dstep = 4.0f * pf;
+            const float ul = ap.inv_wl * A.wrx.s_half, ur = ap.inv_wr * A.wrx.s_half;
             for (int j = 0; j < KI; ++j) {
                 const float d = fmaf((float)j, dstep, dgb);
-                const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
-                float a = window_eval<WCR>(t, A.wrx);
-                a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
+                float a;
+                if (WCR == WC_COS_ALL) {
+                    float u = fabsf(d) * (d < 0.f ? ul : ur);  // revolutions
+                    u = u - rintf(u);
+                    a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
+                } else {
+                    const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
+                    a = window_eval<WCR>(t, A.wrx);
+                    a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
+                }
                 const float rv = rev_of((uint32_t)n * Qrx[j]);
                 const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
                 const float br = a * cs, bi = a * sn;
@@ -524,9 +536,16 @@ void beamform_kernel(BeamArgs A)
                     const int j = 8 * cc + s;
                     const int k = kelem<PREC>(j, g);
                     const float d = fmaf((float)(k - k0), pf, dgb);
-                    const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
-                    float a = window_eval<WCR>(t, A.wrx);
-                    a = (k >= ap.klo && k <= ap.khi) ? a : 0.f;
+                    float a;
+                    if (WCR == WC_COS_ALL) {
+                        float u = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr) * A.wrx.s_half;  // revolutions
+                        u = u - rintf(u);
+                        a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
+                    } else {
+                        const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
+                        a = window_eval<WCR>(t, A.wrx);
+                        a = (k >= ap.klo && k <= ap.khi) ? a : 0.f;
+                    }
                     const float rv = rev_of((uint32_t)n * Qrx[j]);
                     const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
                     br[s] = a * cs;

@@ -61,6 +61,18 @@ class _Plan:
         _lib.check(_lib.load().fn_plan_execute(self._h, _ptr(rf), dt, _ptr(image), _ptr(weights), _ptr(weights_rx),
                                                C.c_void_p(handle)), self.prefix)
 
+    def autotune(self, rf, image, repetitions: int = 3, stream=None) -> dict:
+        """Time the candidate tilings (transmit tiles per workgroup, bin split) on rf/image
+        (image contents are overwritten) and keep the fastest; returns the choice."""
+        r = _lib.TuneResult()
+        _lib.check(_lib.load().fn_plan_autotune(self._h, _ptr(rf), _rf_dtype(rf, self.prefix), _ptr(image),
+                                                int(repetitions), C.byref(r), _stream(stream, rf.device)),
+                   self.prefix)
+        return {f: getattr(r, f) for f, _ in _lib.TuneResult._fields_}
+
+    def set_tiling(self, m_tiles: int, bin_split: int):
+        _lib.check(_lib.load().fn_plan_set_tiling(self._h, int(m_tiles), int(bin_split)), self.prefix)
+
     def set_profiling(self, enable: bool = True):
         _lib.check(_lib.load().fn_plan_set_profiling(self._h, int(bool(enable))), self.prefix)
 
@@ -136,6 +148,8 @@ class PwPlan(_Plan):
         self.n_samples, self.n_elements, self.n_angles = int(n_samples), int(n_elements), len(keep[2])
         self.n_frames = max(1, int(n_frames))
         self.c_0 = float(c_0)
+        self.precision = precision
+        self.window_class = type(window).__name__
         self._rf_numel = self.n_samples * self.n_elements * self.n_angles * self.n_frames
 
     def set_tof(self, c_tof=0.0, delay_offsets=None):
@@ -220,3 +234,46 @@ class SaPlan(_Plan):
         self._keep = keep
         self.grid = info
         self._rf_numel = int(n_samples) * int(n_elements) * int(n_elements)
+
+
+class Batch:
+    """Many plans executed as one hipGraph (fnumber_hip.h fn_batch_*): the launch
+    sequences of all plans are captured once over `n_streams` forked streams and
+    replayed by one graph launch per execute(); a change of any buffer pointer
+    captures again."""
+
+    def __init__(self, plans, n_streams: int = 4):
+        self.plans = list(plans)
+        arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        self._h = C.c_void_p()
+        _lib.check(_lib.load().fn_batch_create(arr, len(self.plans), int(n_streams), C.byref(self._h)), "fn_batch")
+
+    def execute(self, rfs, images, stream=None):
+        n = len(self.plans)
+        if len(rfs) != n or len(images) != n:
+            raise _lib.FNumberError("fn_batch:InvalidArgument", "one RF and one image per plan")
+        for p, rf, img in zip(self.plans, rfs, images):
+            if rf.numel() != p._rf_numel or not rf.is_cuda or not rf.is_contiguous():
+                raise _lib.FNumberError("fn_batch:InvalidRFData", "RF size does not match its plan")
+            if img.numel() != p.info()["n_voxels"] * getattr(p, "n_frames", 1) or not img.is_contiguous():
+                raise _lib.FNumberError("fn_batch:InvalidArgument", "image size does not match its plan")
+        r = (C.c_void_p * n)(*[t.data_ptr() for t in rfs])
+        d = (C.c_int32 * n)(*[_rf_dtype(t, "fn_batch") for t in rfs])
+        im = (C.c_void_p * n)(*[t.data_ptr() for t in images])
+        _lib.check(_lib.load().fn_batch_execute(self._h, r, d, im, _stream(stream, rfs[0].device)), "fn_batch")
+
+    def info(self) -> dict:
+        nodes, caps = C.c_int64(), C.c_int32()
+        _lib.check(_lib.load().fn_batch_info(self._h, C.byref(nodes), C.byref(caps)), "fn_batch")
+        return {"graph_nodes": nodes.value, "captures": caps.value}
+
+    def close(self):
+        if self._h:
+            _lib.load().fn_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -229,6 +229,35 @@ int32_t fn_plan_set_profiling(fn_plan *plan, int32_t enable);
 int32_t fn_plan_get_profile(fn_plan *plan, double *spectra_ms, double *beamform_ms, int32_t *n_runs);
 int32_t fn_plan_destroy(fn_plan *plan);
 
+/* ---- tuning and batches (no counterpart in the reference) ----------------- */
+/* Tiling of a plan's beamforming launch: m_tiles transmit tiles of 16 rows per
+ * workgroup (plane waves 1..5, ensembles 1..4; synthetic aperture fixed) and the
+ * number of bin chunks split across workgroups (bin_split, 1..n_f). */
+typedef struct fn_tune_result {
+    double best_ms;               /* per execution, HIP events on the tuning stream */
+    double default_ms;            /* the plan's initial tiling */
+    int32_t m_tiles, bin_split;   /* chosen (and applied) */
+    int32_t n_candidates;
+} fn_tune_result;
+/* Times the candidate tilings on d_rf / d_image (contents overwritten) and keeps the fastest. */
+int32_t fn_plan_autotune(fn_plan *plan, const void *d_rf, int32_t rf_dtype, float *d_image, int32_t repetitions,
+                         fn_tune_result *result, void *stream);
+/* Applies a tiling (e.g. one stored from an earlier fn_plan_autotune). */
+int32_t fn_plan_set_tiling(fn_plan *plan, int32_t m_tiles, int32_t bin_split);
+
+/* A batch executes many plans (same device) as one hipGraph: the launch
+ * sequences of all plans are captured once, spread over n_streams forked
+ * streams, and replayed by one graph launch per fn_batch_execute.  The graph
+ * is captured again when any buffer pointer differs from the captured ones.
+ * Arrays have one entry per plan; rf_dtypes FN_DTYPE_*; weights may be NULL. */
+typedef struct fn_batch fn_batch;
+int32_t fn_batch_create(fn_plan *const *plans, int32_t n_plans, int32_t n_streams, fn_batch **batch);
+int32_t fn_batch_execute(fn_batch *batch, const void *const *d_rf, const int32_t *rf_dtypes, float *const *d_images,
+                         void *stream);
+/* graph nodes and the number of captures so far */
+int32_t fn_batch_info(const fn_batch *batch, int64_t *n_nodes, int32_t *n_captures);
+int32_t fn_batch_destroy(fn_batch *batch);
+
 /* ---- differentiation form (plane-wave plans) ---------------------------- */
 /* No counterpart in the reference (its MEX only beamforms); these extend the
  * plan of fn_plan_pw_create with the time-of-flight parameters

@@ -112,3 +112,83 @@ def test_doppler_frames_differ_where_the_flow_is(fa, torch):
     z = w0.positions_z
     inside = np.abs(z - 20e-3) < 3e-3
     assert d[:, inside].mean() > 5 * d[:, ~inside].mean()
+
+
+def _cpwc(n_x, n_z, n_ang, seed, n_el=128, n_samples=1024):
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.cpwc_small(), n_x, n_z, n_samples=n_samples, n_speckle=1500, n_elements=n_el)
+    w.angles_deg = np.linspace(-10.0, 10.0, n_ang) if n_ang > 1 else np.zeros(1)
+    w.seed = seed
+    return w
+
+
+@pytest.mark.parametrize("n_ang", [11, 40])
+def test_autotune_keeps_the_image(fa, torch, n_ang):
+    w = _cpwc(48, 40, n_ang, 31 + n_ang)
+    rf = w.rf_device("cuda:0")
+    plan = _plan(fa, w)
+    a = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device="cuda")
+    b = torch.empty_like(a)
+    plan.execute(rf, a)
+    res = plan.autotune(rf, b, repetitions=2)
+    plan.execute(rf, b)
+    torch.cuda.synchronize()
+    info = plan.info()
+    print(f"case autotune n_ang={n_ang}: {res} -> m_tiles {info['m_tiles']} bin_split {info['bin_split']}", flush=True)
+    assert res["n_candidates"] >= 2 and res["best_ms"] <= res["default_ms"] * (1 + 1e-9)
+    assert info["m_tiles"] == res["m_tiles"] and info["bin_split"] == res["bin_split"]
+    assert (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a)).item() < 1e-6
+    with pytest.raises(fa.FNumberError):
+        plan.set_tiling(9, 1)
+    plan.close()
+
+
+def test_batch_graph_replays_match_eager(fa, torch):
+    """One graph over heterogeneous plans (grids, angle counts, precisions, an ensemble
+    and a synthetic-aperture plan); every replay equals the eager images, with the
+    program's own eager work in between; new buffers trigger a new capture."""
+    from f_number_amd import synth
+
+    jobs = []
+    for (nx, nz, na, prec) in [(48, 40, 11, "fp32"), (32, 64, 1, "fp16"), (64, 32, 31, "bf16x3"), (40, 40, 3, "fp32")]:
+        w = _cpwc(nx, nz, na, 100 + nx + na)
+        jobs.append((_plan(fa, w, precision=prec), w.rf_device("cuda:0"), (nx, nz), 1))
+    ws, rfe = _frames(5, 5)
+    jobs.append((_plan(fa, ws[0], precision="fp16", n_frames=5), torch.from_numpy(rfe).cuda(),
+                 (ws[0].positions_x.size, ws[0].positions_z.size), 5))
+    ws = synth.scaled(synth.saft_full(), 20, 16, n_samples=768, n_speckle=300, n_elements=32)
+    ws.pitch = 4 * 0.3048e-3
+    ws.width = 4 * 0.27e-3
+    sa = fa.SaPlan(ws.positions_x, ws.positions_z, ws.n_samples, ws.n_elements, ws.f_s, ws.width, ws.pitch, ws.c_0,
+                   ws.f_bounds, 0.0, output_scale="cpu")
+    jobs.append((sa, ws.rf_device("cuda:0"), (20, 16), 1))
+    eager, outs = [], []
+    for plan, rf, (nx, nz), nfr in jobs:
+        shape = (nfr, nx, nz) if nfr > 1 else (nx, nz)
+        e = torch.empty(shape, dtype=torch.complex64, device="cuda")
+        plan.execute(rf, e)
+        eager.append(e)
+        outs.append(torch.empty_like(e))
+    torch.cuda.synchronize()
+    batch = fa.Batch([j[0] for j in jobs], n_streams=3)
+    rfs = [j[1] for j in jobs]
+    for replay in range(4):
+        for o in outs:
+            o.fill_(float("nan"))  # eager work between replays
+        batch.execute(rfs, outs)
+        torch.cuda.synchronize()
+        for i, (o, e) in enumerate(zip(outs, eager)):
+            err = (torch.linalg.vector_norm(o - e) / torch.linalg.vector_norm(e)).item()
+            assert err < 1e-6, f"replay {replay} job {i}: {err}"
+    info = batch.info()
+    print(f"case batch: {info}", flush=True)
+    assert info["captures"] == 1 and info["graph_nodes"] >= len(jobs) * 2
+    outs[0] = torch.empty_like(outs[0])
+    batch.execute(rfs, outs)
+    torch.cuda.synchronize()
+    assert batch.info()["captures"] == 2
+    assert (torch.linalg.vector_norm(outs[0] - eager[0]) / torch.linalg.vector_norm(eager[0])).item() < 1e-6
+    batch.close()
+    for j in jobs:
+        j[0].close()

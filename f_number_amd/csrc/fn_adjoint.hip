@@ -15,13 +15,15 @@
 // multiplications): per bin, D[element][angle] += sum_v conj(B)[element][v] U[v][angle]
 // with U = s y conj(T) / w.  The contraction index (voxels) is the MFMA K, so
 // lane l holds the pairs (voxel 4s + l/16 of a 16-voxel group, element
-// l%16 + 16 mt).  A block of 8 waves owns 8 consecutive bins (one per wave)
+// l%16 + 16 mt).  A block of 4 waves owns 4 consecutive bins (one per wave)
 // and a voxel range: per 16-voxel group the block computes the phase words
 // (FP64 time of flight reduced to 2^-32 revolutions) once into LDS, every wave
 // contracts the group for its own bin and keeps its 16 x 128 x 16 result in
 // registers across the whole range -- no atomics; one partial per (range, bin)
-// is written at the end and reduced by a second kernel.  Apertures are
-// computed per voxel in the forward layout and exchanged with lane shuffles.
+// is written at the end and reduced by a second kernel.  Apertures: each wave
+// computes those of its bin for 64 voxels at once (one voxel per lane, from
+// per-voxel FP64 terms precomputed with the phase words) every fourth group and
+// keeps them in LDS for the four groups.
 // Phase tiles are recomputed per bin, never stored.
 //
 // Vector-Jacobian products for a cotangent g (J real, dJ = sum Re(conj(g) dI)):
@@ -45,19 +47,45 @@ __device__ __forceinline__ float row_sum16(float v)
     return v;
 }
 
-constexpr int ADJ_WAVES = 8;   // bins per block, one per wave
+// min / max over the 16 lanes of a DPP row, result in every lane of the row
+__device__ __forceinline__ int row_min16(int v)
+{
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));
+    return v;
+}
+
+__device__ __forceinline__ int row_max16(int v)
+{
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));
+    return v;
+}
+
+constexpr int ADJ_WAVES = 4;   // bins per block, one per wave (two blocks per CU)
 constexpr int ADJ_ET = 8;      // element tiles of 16 (128 elements)
 constexpr int ADJ_QS = 132;    // LDS words per voxel of the receive phase table ([cidx][mt] + pad)
 
-template <int WCR>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(AdjArgs S)
+// FULL: no per-element support test on a 128-element array (no padded
+// elements).  Either the support is every element (the test reduces to the
+// aperture's validity, which the voxel scale already carries), or it is the
+// snapped aperture [lb, ub] of a window cut at t = 1 with pitch > width / 2: an
+// element outside a valid snapped aperture then has t > 1 (with W the snapped
+// half-width on its side, t - 1 >= (pitch - width/2) / (h + width/2)), so the
+// window's own cut zeroes it.
+template <int WCR, bool FULL>
+__global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(AdjArgs S)
 {
     // group tables, double-buffered: the next group is loaded while this one is contracted
     __shared__ __align__(16) uint32_t q_rx[2][16 * ADJ_QS];  // [voxel i][cidx][mt]: element cidx + 16 mt
     __shared__ uint32_t q_tx[2][16 * 16];      // [voxel i][angle column]
-    __shared__ double vxz[2][2 * 16];          // x, z of the group's voxels
     __shared__ float2 vy[2][16];               // cotangent of the group's voxels (0 outside the range)
-    __shared__ __align__(16) uint4 apx[ADJ_WAVES][16];  // per wave: aperture (klo, khi, 1/W_L, 1/W_R) per voxel
+    // per wave, for 64 voxels: (klo | khi << 16, 1/W_L, 1/W_R, x_0 - x) with x_0 = -M pitch
+    __shared__ __align__(16) uint4 apx[ADJ_WAVES][64];
     const BeamArgs &A = S.B;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cidx = lane & 15, q = lane >> 4;
@@ -83,70 +111,96 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void a
         K3[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
 
-    const unsigned nz = (unsigned)A.n_z;
     const int qtx_stride = 16 * S.n_at;
     // copy the phase words and voxel data of the group at `base` into buffer `buf`
+    static_assert(ADJ_WAVES == 4, "group loader assumes 256 threads");
     auto load_group = [&](long base, int buf) {
-        const int i = tid >> 5, j4 = tid & 31;  // 512 threads: 16 voxels x 32 uint4
-        const long u = base + i;
-        uint4 w = make_uint4(0u, 0u, 0u, 0u);
-        if (u < n_vox) w = reinterpret_cast<const uint4 *>(S.qrx + (size_t)u * 128)[j4];
-        *reinterpret_cast<uint4 *>(&q_rx[buf][i * ADJ_QS + 4 * j4]) = w;
-        if (tid < 256) {
-            const int iv = tid >> 4, col = tid & 15;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int it = tid + 256 * h, i = it >> 5, j4 = it & 31;  // 16 voxels x 32 uint4
+            const long u = base + i;
+            uint4 w = make_uint4(0u, 0u, 0u, 0u);
+            if (u < n_vox) w = reinterpret_cast<const uint4 *>(S.qrx + (size_t)u * 128)[j4];
+            *reinterpret_cast<uint4 *>(&q_rx[buf][i * ADJ_QS + 4 * j4]) = w;
+        }
+        {
+            const int iv = tid >> 4, col = tid & 15;  // 16 voxels x 16 angle columns
             const long uv = base + iv;
             q_tx[buf][tid] = uv < n_vox ? S.qtx[(size_t)uv * qtx_stride + 16 * at + col] : 0u;
-        } else if (tid < 272) {
-            const int t = tid - 256;
-            const long u2 = base + t;
-            const unsigned uu = (unsigned)min(u2, n_vox - 1);  // n_vox < 2^31
-            const unsigned ix = uu / nz, iz = uu - ix * nz;
-            vxz[buf][2 * t] = A.px[ix];
-            vxz[buf][2 * t + 1] = A.pz[iz];
-            vy[buf][t] = u2 < vr1 ? S.y[u2] : make_float2(0.f, 0.f);
+        }
+        if (tid < 16) {
+            const long u2 = base + tid;
+            vy[buf][tid] = u2 < vr1 ? S.y[u2] : make_float2(0.f, 0.f);
         }
     };
-    if (vr0 < vr1) load_group(vr0, 0);
+    // per-voxel aperture terms of voxel `base + lane` (clamped into the grid)
+    auto load_geo = [&](long base) {
+        const double *g = S.vgeo + 4 * (size_t)min(base + lane, n_vox - 1);
+        return make_double4(g[0], g[1], g[2], g[3]);
+    };
+    const bool a_ok = 16 * at + cidx < A.n_tx;
+    const double x0 = -A.M * A.pitch;
+    double4 geo = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (vr0 < vr1) {
+        load_group(vr0, 0);
+        if (active) geo = load_geo(vr0);
+    }
     __syncthreads();
     int cur = 0;
     for (long base = vr0; base < vr1; base += 16, cur ^= 1) {
         if (base + 16 < vr1) load_group(base + 16, cur ^ 1);  // the buffer last read two groups ago
+        const int gq = (int)((base - vr0) >> 4) & 3;           // group within the 64 voxels of apx
         if (active) {
-            // aperture of this wave's bin for voxel cidx of the group (forward layout)
-            const double xf = vxz[cur][2 * cidx], zf = vxz[cur][2 * cidx + 1];
-            double Fr = Fb;
-            if (A.clamp) Fr = fmin(Fr, sqf * sqrt(zf / (2.88 * A.c0)));  // das_pw.m:374-377
-            const Aperture ap = make_aperture(xf, xf / A.pitch, zf / (2.0 * A.pitch), Fr, A, A.support_rx);
-            const bool a_ok = 16 * at + cidx < A.n_tx;
-            // exchange through LDS: lanes of row q need the apertures of voxels 4s + q
-            if (lane < 16)
-                apx[wave][cidx] = make_uint4((uint32_t)ap.klo, (uint32_t)ap.khi, __float_as_uint(ap.inv_wl),
-                                             __float_as_uint(ap.inv_wr));
+            if (gq == 0) {
+                // apertures of this wave's bin for voxels base .. base + 63, one per lane (forward formulas)
+                double Fr = Fb;
+                if (A.clamp) Fr = fmin(Fr, sqf * geo.w);  // das_pw.m:374-377
+                const Aperture ap = make_aperture(geo.x, geo.y, geo.z, Fr, A, A.support_rx);
+                apx[wave][lane] = make_uint4((uint32_t)ap.klo | ((uint32_t)ap.khi << 16), __float_as_uint(ap.inv_wl),
+                                             __float_as_uint(ap.inv_wr), __float_as_uint((float)(x0 - geo.x)));
+                if (base + 64 < vr1) geo = load_geo(base + 64);  // consumed four groups later
+            }
+
+            // element tiles [t_lo, t_hi] of 16 that meet the support of some voxel of the
+            // group (wave-uniform): every other tile has zero window for all 16 voxels
+            int t_lo, t_hi;
+            {
+                const uint32_t kk = apx[wave][16 * gq + cidx].x;
+                const int kl = (int)(kk & 0xffffu), kh = (int)(kk >> 16);
+                const int lo = row_min16(kl <= kh ? kl : 0x7fff), hi = row_max16(kl <= kh ? kh : -1);
+                t_lo = __builtin_amdgcn_readfirstlane(lo) >> 4;
+                t_hi = __builtin_amdgcn_readfirstlane(hi) >> 4;  // -1: no voxel of the group has elements
+            }
 
             // Gauss: conj(B) U = (br - j bi)(ur + j ui):
             //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = -bi (ur + ui); Re = K1 - K3, Im = K1 + K2
     #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int i = 4 * s + q;  // voxel of this lane in step s
-                const uint4 apv = apx[wave][i];  // written by this wave above (LDS operations of a wave are ordered)
-                const int klo = (int)apv.x, khi = (int)apv.y;
-                const float iwl = __uint_as_float(apv.z), iwr = __uint_as_float(apv.w);
-                const float dg = (float)(((double)cidx - A.M) * A.pitch - vxz[cur][2 * i]);
+                const uint4 apv = apx[wave][16 * gq + i];  // written by this wave (LDS operations of a wave are ordered)
+                const int klo = (int)(apv.x & 0xffffu), khi = (int)(apv.x >> 16);
+                const float iwl = __uint_as_float(apv.y), iwr = __uint_as_float(apv.z);
+                const float dg = fmaf((float)cidx, pf, __uint_as_float(apv.w));  // x_cidx - x
                 float wv[ADJ_ET];
                 float asum = 0.f;
     #pragma unroll
                 for (int mt = 0; mt < ADJ_ET; ++mt) {
+                    if (mt < t_lo || mt > t_hi) {
+                        wv[mt] = 0.f;
+                        continue;
+                    }
                     const int e = cidx + 16 * mt;
                     const float d = fmaf((float)(16 * mt), pf, dg);
                     const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
                     float w = window_eval<WCR>(t, A.wrx);
-                    w = (e >= klo && e <= khi) ? w : 0.f;
+                    if (!FULL) w = (e >= klo && e <= khi) ? w : 0.f;
                     wv[mt] = w;
                     asum += w;
                 }
                 // weight sum over the 128 elements of voxel i: 16 lanes of the row (DPP)
                 asum = row_sum16(asum);
-                const float sc = (klo <= khi && a_ok) ? A.out_scale * (A.norm_rx ? 1.0f / asum : 1.0f) : 0.f;
+                const float sc =
+                    (klo <= khi && a_ok) ? A.out_scale * (A.norm_rx ? __builtin_amdgcn_rcpf(asum) : 1.0f) : 0.f;
                 // U = s y conj(T) / w
                 const float2 yv = vy[cur][i];
                 const float rt = rev_of(n * q_tx[cur][i * 16 + cidx]);
@@ -159,6 +213,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void a
                 const uint32_t Q[ADJ_ET] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
     #pragma unroll
                 for (int mt = 0; mt < ADJ_ET; ++mt) {
+                    if (mt < t_lo || mt > t_hi) continue;
                     const float rv = rev_of(n * Q[mt]);
                     const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
                     const float br = wv[mt] * cs, bi = wv[mt] * sn;
@@ -190,17 +245,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void a
 // frac(f_s tau / N) in units of 2^-32.  qrx[v][cidx][mt] holds element
 // cidx + 16 mt (zero-padded to 128 elements), qtx[v][a] angle a (16 n_at columns).
 __global__ __launch_bounds__(256) void adjoint_tables_kernel(BeamArgs A, uint32_t *__restrict__ qrx,
-                                                             uint32_t *__restrict__ qtx, int n_at)
+                                                             uint32_t *__restrict__ qtx, double *__restrict__ vgeo,
+                                                             int n_at)
 {
     const long n_vox = (long)A.n_x * A.n_z;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    const int n_cols = 128 + 16 * n_at;
+    const int n_cols = 128 + 16 * n_at + 1;
     if (i >= n_vox * n_cols) return;
     const long v = i / n_cols;
     const int col = (int)(i - v * n_cols);
     const int ix = (int)(v / A.n_z), iz = (int)(v - (long)ix * A.n_z);
     const double x = A.px[ix], z = A.pz[iz];
-    if (col < 128) {
+    if (col == 128 + 16 * n_at) {
+        // aperture terms, the same expressions as the forward kernel's prologue
+        vgeo[4 * v] = x;
+        vgeo[4 * v + 1] = x / A.pitch;
+        vgeo[4 * v + 2] = z / (2.0 * A.pitch);
+        vgeo[4 * v + 3] = sqrt(z / (2.88 * A.c0));
+    } else if (col < 128) {
         const int e = col;
         const double lat = (((double)e - A.M) * A.pitch - x) / A.ctof, zc = z / A.ctof;
         const double tau = sqrt(lat * lat + zc * zc) + (e < A.n_el ? A.delay[e] : 0.0);
@@ -324,14 +386,25 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restr
 hipError_t launch_adjoint(int wcr, dim3 grid, hipStream_t s, const AdjArgs &S)
 {
     const long n_vox = (long)S.B.n_x * S.B.n_z;
-    const long n_tab = n_vox * (128 + 16 * S.n_at);
+    const long n_tab = n_vox * (128 + 16 * S.n_at + 1);
     hipLaunchKernelGGL(adjoint_tables_kernel, dim3((unsigned)((n_tab + 255) / 256)), dim3(256), 0, s, S.B,
-                       const_cast<uint32_t *>(S.qrx), const_cast<uint32_t *>(S.qtx), S.n_at);
+                       const_cast<uint32_t *>(S.qrx), const_cast<uint32_t *>(S.qtx), const_cast<double *>(S.vgeo),
+                       S.n_at);
     const dim3 block(64 * ADJ_WAVES);
+    const bool full = range_free(S.B, 16 * ADJ_ET);
     switch (wcr) {
-    case WC_RANGE: hipLaunchKernelGGL(adjoint_kernel<WC_RANGE>, grid, block, 0, s, S); break;
-    case WC_LINEAR: hipLaunchKernelGGL(adjoint_kernel<WC_LINEAR>, grid, block, 0, s, S); break;
-    default: hipLaunchKernelGGL(adjoint_kernel<WC_COS>, grid, block, 0, s, S); break;
+    case WC_RANGE:
+        if (full) hipLaunchKernelGGL((adjoint_kernel<WC_RANGE, true>), grid, block, 0, s, S);
+        else hipLaunchKernelGGL((adjoint_kernel<WC_RANGE, false>), grid, block, 0, s, S);
+        break;
+    case WC_LINEAR:
+        if (full) hipLaunchKernelGGL((adjoint_kernel<WC_LINEAR, true>), grid, block, 0, s, S);
+        else hipLaunchKernelGGL((adjoint_kernel<WC_LINEAR, false>), grid, block, 0, s, S);
+        break;
+    default:
+        if (full) hipLaunchKernelGGL((adjoint_kernel<WC_COS, true>), grid, block, 0, s, S);
+        else hipLaunchKernelGGL((adjoint_kernel<WC_COS, false>), grid, block, 0, s, S);
+        break;
     }
     return hipGetLastError();
 }

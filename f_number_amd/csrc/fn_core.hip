@@ -696,22 +696,23 @@ int execute_tangent(fn_plan *P, const void *d_rf, int dtype, float *d_image, flo
 int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
 {
     const int n_f = P->g.n_f;
-    const int n_chunks = (n_f + 7) / 8;  // bins per block of the contraction kernel (one per wave)
+    const int n_chunks = (n_f + 3) / 4;  // bins per block of the contraction kernel (one per wave)
     if (!P->d_Z) {
         P->adj_at = (P->n_tx + 15) / 16;
-        const long groups16 = (P->n_vox + 15) / 16;
-        // about two rounds of one block per CU on 256 CUs (rounded down so the
-        // last round is not a short tail), bounded by the voxel groups
-        long vr = 512 / ((long)n_chunks * P->adj_at);
-        vr = std::max(1L, std::min(vr, groups16));
-        const long groups_per_range = (groups16 + vr - 1) / vr;
-        P->adj_vpr = groups_per_range * 16;
-        P->adj_vr = (int)((groups16 + groups_per_range - 1) / groups_per_range);
+        const long groups64 = (P->n_vox + 63) / 64;  // ranges are whole 64-voxel aperture blocks
+        // about four rounds of two blocks per CU on 256 CUs (rounded down so the
+        // last round is not a short tail), bounded by the voxel blocks
+        long vr = 2048 / ((long)n_chunks * P->adj_at);
+        vr = std::max(1L, std::min(vr, groups64));
+        const long blocks_per_range = (groups64 + vr - 1) / vr;
+        P->adj_vpr = blocks_per_range * 64;
+        P->adj_vr = (int)((groups64 + blocks_per_range - 1) / blocks_per_range);
         const size_t p_floats = (size_t)P->adj_vr * n_f * P->adj_at * 2 * 128 * 16;
         HIP_CHECK(hipMalloc(&P->d_P, p_floats * sizeof(float)), "adjoint partials");
         HIP_CHECK(hipMalloc(&P->d_Z, (size_t)n_f * P->n_tx * P->n_el * sizeof(float2)), "adjoint contraction");
         HIP_CHECK(hipMalloc(&P->d_part, 1024 * sizeof(double)), "reduction partials");
-        HIP_CHECK(hipMalloc(&P->d_qtab, (size_t)P->n_vox * (128 + 16 * P->adj_at) * sizeof(uint32_t)),
+        // phase words (128 + 16 n_at per voxel) followed by 4 doubles of aperture terms per voxel
+        HIP_CHECK(hipMalloc(&P->d_qtab, (size_t)P->n_vox * (128 + 16 * P->adj_at + 8) * sizeof(uint32_t)),
                   "phase tables");
     }
     fnk::AdjArgs S{};
@@ -720,6 +721,7 @@ int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
     S.P = P->d_P;
     S.qrx = P->d_qtab;
     S.qtx = P->d_qtab + (size_t)P->n_vox * 128;
+    S.vgeo = reinterpret_cast<const double *>(P->d_qtab + (size_t)P->n_vox * (128 + 16 * P->adj_at));
     S.vox_per_range = P->adj_vpr;
     S.n_at = P->adj_at;
     const dim3 grid((unsigned)P->adj_vr, (unsigned)n_chunks, (unsigned)P->adj_at);

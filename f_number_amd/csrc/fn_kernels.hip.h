@@ -93,7 +93,8 @@ struct AdjArgs {
     float *P;               // partials [n_vr][n_f][n_at][2][128][16]
     const uint32_t *qrx;    // receive phase words [n_vox][cidx 16][mt 8] (element cidx + 16 mt)
     const uint32_t *qtx;    // transmit phase words [n_vox][16 n_at]
-    long vox_per_range;     // multiple of 16
+    const double *vgeo;     // per voxel: x, x / pitch, z / (2 pitch), sqrt(z / (2.88 c0))
+    long vox_per_range;     // multiple of 64
     int n_at;               // angle tiles of 16
 };
 
@@ -157,6 +158,18 @@ __device__ __forceinline__ Aperture make_aperture(double x, double xp, double zp
         }
     }
     return ap;
+}
+
+// The receive window needs no per-element support test on an array of
+// n_full elements (no padding): support on every element, or the snapped
+// aperture of a window cut at t = 1 when pitch > width / 2 (elements outside a
+// valid snapped aperture have t - 1 >= (pitch - width/2) / (h + width/2) > 0;
+// the 1e-3 margin keeps that far above float rounding).
+inline bool range_free(const BeamArgs &A, int n_full)
+{
+    if (A.n_el != n_full) return false;
+    if (A.support_rx == SUPPORT_ALL) return true;
+    return A.support_rx == SUPPORT_SNAPPED && A.wrx.t_cut <= 1.0f && 0.5 * A.width <= 0.999 * A.pitch;
 }
 
 template <int WC>

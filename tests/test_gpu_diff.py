@@ -83,8 +83,9 @@ def _img_dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a.T).astype(np.complex64)).cuda()
 
 
+# (128, tukey, cpu): every element in support on a full 128-element array (the adjoint's FULL variant)
 CFG = [(64, 3, "hann", "cpu", "on"), (128, 11, "tukey", "mex", "on"), (32, 40, "boxcar", "cpu", "off"),
-       (100, 20, "triangular", "cpu", "off")]
+       (100, 20, "triangular", "cpu", "off"), (128, 5, "tukey", "cpu", "on")]
 
 
 @pytest.mark.parametrize("n_el,n_ang,win,sem,norm", CFG)

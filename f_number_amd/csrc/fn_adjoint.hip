@@ -36,36 +36,6 @@
 
 namespace fnk {
 
-// sum over the 16 lanes of a DPP row, result in every lane of the row:
-// quad xor 1, quad xor 2, half-row mirror (quad <-> quad), row mirror (half <-> half)
-__device__ __forceinline__ float row_sum16(float v)
-{
-    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
-    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
-    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, true));  // row_half_mirror
-    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, true));  // row_mirror
-    return v;
-}
-
-// min / max over the 16 lanes of a DPP row, result in every lane of the row
-__device__ __forceinline__ int row_min16(int v)
-{
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));
-    return v;
-}
-
-__device__ __forceinline__ int row_max16(int v)
-{
-    v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
-    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
-    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
-    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));
-    return v;
-}
-
 constexpr int ADJ_WAVES = 4;   // bins per block, one per wave (two blocks per CU)
 constexpr int ADJ_ET = 8;      // element tiles of 16 (128 elements)
 constexpr int ADJ_QS = 132;    // LDS words per voxel of the receive phase table ([cidx][mt] + pad)
@@ -173,7 +143,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
             }
 
             // Gauss: conj(B) U = (br - j bi)(ur + j ui):
-            //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = -bi (ur + ui); Re = K1 - K3, Im = K1 + K2
+            //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = bi (ur + ui); Re = K1 + K3, Im = K1 + K2
     #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int i = 4 * s + q;  // voxel of this lane in step s
@@ -219,7 +189,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
                     const float br = wv[mt] * cs, bi = wv[mt] * sn;
                     K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br - bi, ur, K1[mt], 0, 0, 0);
                     K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, u2, K2[mt], 0, 0, 0);
-                    K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(-bi, u3, K3[mt], 0, 0, 0);
+                    K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi, u3, K3[mt], 0, 0, 0);
                 }
             }
         }  // active
@@ -235,7 +205,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int o = (16 * mt + 4 * q + r) * 16 + cidx;
-            dst[o] = K1[mt][r] - K3[mt][r];
+            dst[o] = K1[mt][r] + K3[mt][r];
             dst[ADJ_ET * 256 + o] = K1[mt][r] + K2[mt][r];
         }
 }
@@ -278,17 +248,19 @@ __global__ __launch_bounds__(256) void adjoint_tables_kernel(BeamArgs A, uint32_
     }
 }
 
-// Z[b][a][k] = sum over voxel ranges of the partials (one thread per output)
+// Z[b][a][k] = sum over voxel ranges of the partials; one thread per (bin, angle
+// tile, element, angle column), the column fastest so the partial reads coalesce
 __global__ __launch_bounds__(256) void adjoint_reduce_kernel(const float *__restrict__ P, float2 *__restrict__ Z,
                                                              int n_vr, int n_f, int n_at, int n_tx, int n_el)
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    const long total = (long)n_f * n_tx * n_el;
+    const long total = (long)n_f * n_at * 16 * ADJ_ET * 16;
     if (i >= total) return;
-    const int k = (int)(i % n_el);
-    const long ba = i / n_el;
-    const int a = (int)(ba % n_tx), b = (int)(ba / n_tx);
-    const int at = a >> 4, col = a & 15;
+    const int col = (int)(i & 15), k = (int)((i >> 4) % (16 * ADJ_ET));
+    const long bat = i / (16 * 16 * ADJ_ET);  // b n_at + at
+    const int at = (int)(bat % n_at), b = (int)(bat / n_at);
+    const int a = 16 * at + col;
+    if (a >= n_tx || k >= n_el) return;
     const int per_bin = 2 * ADJ_ET * 256;
     const size_t off_re = (size_t)k * 16 + col, off_im = (size_t)ADJ_ET * 256 + off_re;
     float re = 0.f, im = 0.f;
@@ -297,12 +269,16 @@ __global__ __launch_bounds__(256) void adjoint_reduce_kernel(const float *__rest
         re += P[base + off_re];
         im += P[base + off_im];
     }
-    Z[i] = make_float2(re, im);
+    Z[((size_t)b * n_tx + a) * n_el + k] = make_float2(re, im);
 }
 
 // (A^H y)[ch][t] = Re sum_b Z[b][ch] e^{+j 2 pi (n_lb + b) t / N}, ch = a n_el + k.
-// blockIdx.y = channel, blockIdx.x * 256 + tid = sample; twiddles e^{+j 2 pi m / N} in LDS,
-// index (n t) mod N advanced exactly in integers.
+// blockIdx.y = group of IDFT_CH consecutive channels (the Z reads of a bin are
+// uniform across the block), blockIdx.x * 256 + tid = sample; twiddles
+// e^{+j 2 pi m / N} in LDS, index (n t) mod N advanced exactly in integers and
+// shared by the channels of a thread.
+constexpr int IDFT_CH = 8;
+
 __global__ __launch_bounds__(256) void adjoint_idft_kernel(const float2 *__restrict__ Z, const float2 *__restrict__ tw,
                                                            float *__restrict__ out, int n_t, int n_ch, int N, int n_lb,
                                                            int n_f)
@@ -310,20 +286,29 @@ __global__ __launch_bounds__(256) void adjoint_idft_kernel(const float2 *__restr
     extern __shared__ float2 tws[];
     for (int m = threadIdx.x; m < N; m += 256) tws[m] = tw[m];
     __syncthreads();
-    const int ch = blockIdx.y;
+    const int ch0 = blockIdx.y * IDFT_CH;
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= n_t) return;
+    const int n_here = min(IDFT_CH, n_ch - ch0);
     long m = ((long)n_lb * t) % N;
-    float acc = 0.f;
+    float acc[IDFT_CH];
+#pragma unroll
+    for (int c = 0; c < IDFT_CH; ++c) acc[c] = 0.f;
     for (int b = 0; b < n_f; ++b) {
-        const float2 z = Z[(size_t)b * n_ch + ch];
         const float2 w = tws[m];
-        acc = fmaf(z.x, w.x, acc);
-        acc = fmaf(-z.y, w.y, acc);
+        const float2 *zb = Z + (size_t)b * n_ch + ch0;
+#pragma unroll
+        for (int c = 0; c < IDFT_CH; ++c) {
+            const float2 z = c < n_here ? zb[c] : make_float2(0.f, 0.f);
+            acc[c] = fmaf(z.x, w.x, acc[c]);
+            acc[c] = fmaf(-z.y, w.y, acc[c]);
+        }
         m += t;
         if (m >= N) m -= N;
     }
-    out[(size_t)ch * n_t + t] = acc;
+#pragma unroll
+    for (int c = 0; c < IDFT_CH; ++c)
+        if (c < n_here) out[(size_t)(ch0 + c) * n_t + t] = acc[c];
 }
 
 // d J / d delay_k = sum_b w_b sum_a (Xr Zi - Xi Zr); one block per element, FP64 sums.
@@ -383,15 +368,23 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restr
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-hipError_t launch_adjoint(int wcr, dim3 grid, hipStream_t s, const AdjArgs &S)
+hipError_t launch_adjoint(int wcr, bool tables, dim3 grid, hipStream_t s, const AdjArgs &S)
 {
     const long n_vox = (long)S.B.n_x * S.B.n_z;
     const long n_tab = n_vox * (128 + 16 * S.n_at + 1);
-    hipLaunchKernelGGL(adjoint_tables_kernel, dim3((unsigned)((n_tab + 255) / 256)), dim3(256), 0, s, S.B,
-                       const_cast<uint32_t *>(S.qrx), const_cast<uint32_t *>(S.qtx), const_cast<double *>(S.vgeo),
-                       S.n_at);
+    if (tables)
+        hipLaunchKernelGGL(adjoint_tables_kernel, dim3((unsigned)((n_tab + 255) / 256)), dim3(256), 0, s, S.B,
+                           const_cast<uint32_t *>(S.qrx), const_cast<uint32_t *>(S.qtx), const_cast<double *>(S.vgeo),
+                           S.n_at);
     const dim3 block(64 * ADJ_WAVES);
     const bool full = range_free(S.B, 16 * ADJ_ET);
+    // the forward's element-loop classes: WC_COS_RF holds when the support test is
+    // free here too (128 elements); WC_COS_ALL evaluates as the general cosine
+    if (wcr == WC_COS_RF && full) {
+        hipLaunchKernelGGL((adjoint_kernel<WC_COS_RF, true>), grid, block, 0, s, S);
+        return hipGetLastError();
+    }
+    if (wcr == WC_COS_RF || wcr == WC_COS_ALL) wcr = WC_COS;
     switch (wcr) {
     case WC_RANGE:
         if (full) hipLaunchKernelGGL((adjoint_kernel<WC_RANGE, true>), grid, block, 0, s, S);
@@ -412,7 +405,7 @@ hipError_t launch_adjoint(int wcr, dim3 grid, hipStream_t s, const AdjArgs &S)
 hipError_t launch_adjoint_reduce(const float *P, float2 *Z, int n_vr, int n_f, int n_at, int n_tx, int n_el,
                                  hipStream_t s)
 {
-    const long total = (long)n_f * n_tx * n_el;
+    const long total = (long)n_f * n_at * 16 * ADJ_ET * 16;
     hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P, Z, n_vr, n_f,
                        n_at, n_tx, n_el);
     return hipGetLastError();
@@ -429,8 +422,8 @@ hipError_t launch_adjoint_idft(const float2 *Z, const float2 *tw, float *out, in
         if (e != hipSuccess) return e;
         lds_done = lds;
     }
-    hipLaunchKernelGGL(adjoint_idft_kernel, dim3((unsigned)((n_t + 255) / 256), (unsigned)n_ch), dim3(256), lds, s, Z,
-                       tw, out, n_t, n_ch, N, n_lb, n_f);
+    hipLaunchKernelGGL(adjoint_idft_kernel, dim3((unsigned)((n_t + 255) / 256), (unsigned)((n_ch + IDFT_CH - 1) / IDFT_CH)),
+                       dim3(256), lds, s, Z, tw, out, n_t, n_ch, N, n_lb, n_f);
     return hipGetLastError();
 }
 

@@ -50,6 +50,7 @@ static hipError_t launch_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, const
     case WC_RANGE: return launch_one<MT, WC_RANGE, WC_RANGE>(grid, lds, s, A);
     case WC_LINEAR: return launch_one<MT, WC_LINEAR, WC_RANGE>(grid, lds, s, A);
     case WC_COS_ALL: return launch_one<MT, WC_COS_ALL, WC_RANGE>(grid, lds, s, A);
+    case WC_COS_RF: return launch_one<MT, WC_COS_RF, WC_RANGE>(grid, lds, s, A);
     default: return launch_one<MT, WC_COS, WC_RANGE>(grid, lds, s, A);
     }
 }
@@ -108,6 +109,7 @@ static hipError_t launch_ens_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, c
     case WC_RANGE: return launch_ens_one<MT, WC_RANGE>(grid, lds, s, A);
     case WC_LINEAR: return launch_ens_one<MT, WC_LINEAR>(grid, lds, s, A);
     case WC_COS_ALL: return launch_ens_one<MT, WC_COS_ALL>(grid, lds, s, A);
+    case WC_COS_RF: return launch_ens_one<MT, WC_COS_RF>(grid, lds, s, A);
     default: return launch_ens_one<MT, WC_COS>(grid, lds, s, A);
     }
 }
@@ -119,7 +121,9 @@ hipError_t FN_ENS_LAUNCHER(INST_KI, INST_PREC)(int mt, int wcr, dim3 grid, size_
     case 1: return launch_ens_mt<1>(wcr, grid, lds, s, A);
     case 2: return launch_ens_mt<2>(wcr, grid, lds, s, A);
     case 3: return launch_ens_mt<3>(wcr, grid, lds, s, A);
+#if INST_PREC == 0 || INST_PREC == 3  // FP32, FP16; the split forms stop at 3 tiles (register file)
     case 4: return launch_ens_mt<4>(wcr, grid, lds, s, A);
+#endif
     default: return hipErrorInvalidValue;
     }
 }
@@ -134,10 +138,7 @@ hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, 
     case 2: return launch_mt<2>(wcr, grid, lds, s, A);
     case 3: return launch_mt<3>(wcr, grid, lds, s, A);
     case 4: return launch_mt<4>(wcr, grid, lds, s, A);
-    case 5: return launch_mt<5>(wcr, grid, lds, s, A);
-    case 6: return launch_mt<6>(wcr, grid, lds, s, A);
-    case 7: return launch_mt<7>(wcr, grid, lds, s, A);
-    case 8: return launch_mt<8>(wcr, grid, lds, s, A);
+    case 5: return launch_mt<5>(wcr, grid, lds, s, A);  // at most 5 tiles per group (tile_cap)
     default: return hipErrorInvalidValue;
     }
 }
@@ -162,6 +163,7 @@ hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, 
     case WC_RANGE: return launch_wct<WC_RANGE>(wct, grid, lds, s, A);
     case WC_LINEAR: return launch_wct<WC_LINEAR>(wct, grid, lds, s, A);
     case WC_COS_ALL: return launch_wct<WC_COS_ALL>(wct, grid, lds, s, A);
+    case WC_COS_RF: return launch_wct<WC_COS_RF>(wct, grid, lds, s, A);
     default: return launch_wct<WC_COS>(wct, grid, lds, s, A);
     }
 }

@@ -46,7 +46,7 @@ static const EnsLauncher kEnsLaunchers[3][4] = {
 hipError_t launch_beamform_tan_k8(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k16(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k32(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
-hipError_t launch_adjoint(int wcr, dim3 grid, hipStream_t s, const AdjArgs &S);
+hipError_t launch_adjoint(int wcr, bool tables, dim3 grid, hipStream_t s, const AdjArgs &S);
 hipError_t launch_adjoint_reduce(const float *P, float2 *Z, int n_vr, int n_f, int n_at, int n_tx, int n_el,
                                  hipStream_t s);
 hipError_t launch_adjoint_idft(const float2 *Z, const float2 *tw, float *out, int n_t, int n_ch, int N, int n_lb,
@@ -109,6 +109,9 @@ struct fn_plan {
     uint32_t *d_qtab = nullptr; // per-voxel phase words of the contraction
     double *d_part = nullptr;   // reduction partials
     int adj_vr = 0, adj_at = 0;
+    long tof_serial = 1;                // bumped by every set_tof
+    long adj_tab_serial = 0;            // tof_serial the phase tables were built for
+    hipStream_t adj_tab_stream = nullptr;  // stream that built them (reuse is ordered only there)
     long adj_vpr = 0;
     size_t x_bytes = 0, xlp_bytes = 0;
     hipStream_t stream = nullptr;
@@ -149,7 +152,7 @@ int select_device(int index)
 int setup_window(const fn_window &w, int semantics, fnk::WinCoef *c, int *wc, int *support)
 {
     const bool cpu = semantics == FN_SEMANTICS_CPU;
-    *c = fnk::WinCoef{-1.f, INFINITY, 0.f, 1.f, 0.f, 0.f, 0.f};
+    *c = fnk::WinCoef{-1.f, INFINITY, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     switch (w.id) {
     case FN_WINDOW_BOXCAR:
         *wc = fnk::WC_RANGE;
@@ -161,6 +164,8 @@ int setup_window(const fn_window &w, int semantics, fnk::WinCoef *c, int *wc, in
         c->t_cut = 1.f;
         c->A = c->B = 0.5f;
         c->s_half = (float)(0.5 / w.parameter);
+        c->q_s = (float)(0.25 / w.parameter);
+        c->q_o = (float)(-(1.0 - w.parameter) * 0.25 / w.parameter);
         *support = fnk::SUPPORT_SNAPPED;  // the window is zero beyond the snapped aperture
         return FN_OK;
     case FN_WINDOW_HANN:
@@ -168,6 +173,7 @@ int setup_window(const fn_window &w, int semantics, fnk::WinCoef *c, int *wc, in
         c->t_cut = cpu ? INFINITY : 1.f;
         c->A = c->B = 0.5f;
         c->s_half = 0.5f;
+        c->q_s = 0.25f;
         *support = cpu ? fnk::SUPPORT_ALL : fnk::SUPPORT_SNAPPED;
         return FN_OK;
     case FN_WINDOW_TRIANGULAR:
@@ -183,17 +189,33 @@ int setup_window(const fn_window &w, int semantics, fnk::WinCoef *c, int *wc, in
     }
 }
 
-// the raised cosine over every element of a full array (CPU Hann) takes the
-// short element loop (WC_COS_ALL); tangent and adjoint use the general form
+// forward element loops without per-element support tests: the raised cosine
+// over every element of a full array (CPU Hann) takes WC_COS_ALL, a cosine
+// window whose cut implies the support takes WC_COS_RF; tangent and adjoint
+// launch the general classes
 void use_cos_all(fn_plan *P)
 {
     const fnk::WinCoef &w = P->args.wrx;
-    if (P->wcr == fnk::WC_COS && P->args.support_rx == fnk::SUPPORT_ALL && P->n_el == 4 * P->KI && w.t_off == 0.f &&
-        w.t_flat < 0.f && std::isinf(w.t_cut))
+    if (P->wcr != fnk::WC_COS) return;
+    if (P->args.support_rx == fnk::SUPPORT_ALL && P->n_el == 4 * P->KI && w.t_off == 0.f && w.t_flat < 0.f &&
+        std::isinf(w.t_cut))
         P->wcr = fnk::WC_COS_ALL;
+    else if (fnk::range_free(P->args, 4 * P->KI) && w.t_cut == 1.f && w.A == 0.5f && w.B == 0.5f &&
+             (w.t_flat == w.t_off || (w.t_flat < 0.f && w.t_off == 0.f)) && w.q_s > 0.f &&
+             std::isfinite(w.q_s) && std::isfinite(w.q_o))
+        P->wcr = fnk::WC_COS_RF;
 }
 
-int wcr_general(int wcr) { return wcr == fnk::WC_COS_ALL ? fnk::WC_COS : wcr; }
+int wcr_general(int wcr) { return (wcr == fnk::WC_COS_ALL || wcr == fnk::WC_COS_RF) ? fnk::WC_COS : wcr; }
+
+// transmit tiles per plane-wave workgroup that fit the register file without
+// spills: ensembles hold per-row image accumulators, the split 16-bit forms
+// their residual operands on top of that
+int tile_cap(int n_frames, int precision)
+{
+    if (n_frames <= 1) return 5;
+    return (precision == FN_PRECISION_BF16X3 || precision == FN_PRECISION_FP16X3) ? 3 : 4;
+}
 
 int ki_of(int n_el)
 {
@@ -249,7 +271,7 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     if (P->mode == 0) {
         // at most 5 transmit tiles per group keep the LDS tile within 80 KiB
         // (ensembles: 4, for the per-row image accumulators)
-        const int max_tiles = P->n_frames > 1 ? 4 : 5;
+        const int max_tiles = tile_cap(P->n_frames, c.precision);
         const int groups = (P->MT_all + max_tiles - 1) / max_tiles;
         P->MT_tmpl = (P->MT_all + groups - 1) / groups;
         P->n_groups = (P->MT_all + P->MT_tmpl - 1) / P->MT_tmpl;
@@ -550,7 +572,7 @@ size_t dtype_size(int dtype) { return dtype == FN_DTYPE_F64 ? 8 : 4; }
 int max_tiles(const fn_plan *P)
 {
     if (P->mode == 1) return P->KI / 4;
-    return std::min(P->MT_all, P->n_frames > 1 ? 4 : 5);
+    return std::min(P->MT_all, tile_cap(P->n_frames, P->precision));
 }
 
 int set_tiling(fn_plan *P, int mt, int split)
@@ -644,9 +666,11 @@ int set_tof(fn_plan *P, double c_tof, const double *delays)
             d[k] = delays[k];
         }
     HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
-    HIP_CHECK(hipStreamSynchronize(P->stream), "hipStreamSynchronize");
+    // kernels of earlier calls on any stream may still read the delay offsets
+    HIP_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
     HIP_CHECK(hipMemcpy(P->d_delay, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice), "delay offsets");
     P->args.ctof = c_tof > 0 ? c_tof : P->args.c0;
+    ++P->tof_serial;
     return FN_OK;
 }
 
@@ -725,7 +749,14 @@ int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
     S.vox_per_range = P->adj_vpr;
     S.n_at = P->adj_at;
     const dim3 grid((unsigned)P->adj_vr, (unsigned)n_chunks, (unsigned)P->adj_at);
-    hipError_t e = fnk::launch_adjoint(wcr_general(P->wcr), grid, s, S);
+    // the phase tables depend on the geometry and the time of flight only: rebuilt
+    // after set_tof, or when this call's stream is not the one that built them
+    const bool tables = P->adj_tab_serial != P->tof_serial || P->adj_tab_stream != s;
+    hipError_t e = fnk::launch_adjoint(P->wcr, tables, grid, s, S);
+    if (e == hipSuccess && tables) {
+        P->adj_tab_serial = P->tof_serial;
+        P->adj_tab_stream = s;
+    }
     if (e != hipSuccess) return hip_fail(e, "adjoint_kernel");
     e = fnk::launch_adjoint_reduce(P->d_P, P->d_Z, P->adj_vr, n_f, P->adj_at, P->n_tx, P->n_el, s);
     if (e != hipSuccess) return hip_fail(e, "adjoint_reduce_kernel");

@@ -39,7 +39,12 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // part and no cut (CPU-semantics Hann, das_pw.m:449-452 with hann.m:62) on a
 // full array (n_el = 4 KI): no support test, the 1/2 of the argument folded
 // into the per-bin reciprocals.
-enum { WC_RANGE = 0, WC_LINEAR = 1, WC_COS = 2, WC_COS_ALL = 3 };
+// WC_COS_RF: the Hann / Tukey window cut at t = 1 where that cut already zeroes
+// every element outside the support (range_free()), so no per-element support
+// test; evaluated as cos^2 of a clamped quarter-period argument:
+//   1/2 + 1/2 cos(2 pi u) = cos^2(pi u),  u = clamp((t - t_off) s_half, 0, 1/2),
+// which is 1 on the flat part (u = 0) and 0 from the cut on (u = 1/2).
+enum { WC_RANGE = 0, WC_LINEAR = 1, WC_COS = 2, WC_COS_ALL = 3, WC_COS_RF = 4 };
 enum { MODE_PW = 0, MODE_SA = 1 };
 
 // Generic |t| window: 1 for t <= t_flat; for t_flat < t < t_cut:
@@ -52,6 +57,7 @@ enum { MODE_PW = 0, MODE_SA = 1 };
 //   boxcar:       constant 1 on its integer support (range class)
 struct WinCoef {
     float t_flat, t_cut, t_off, A, B, C, s_half;  // cosine argument: (t - t_off) * s_half revolutions
+    float q_s, q_o;  // WC_COS_RF: quarter-period argument t q_s + q_o = (t - t_off) s_half / 2
 };
 
 // element support of a window for one aperture
@@ -160,6 +166,36 @@ __device__ __forceinline__ Aperture make_aperture(double x, double xp, double zp
     return ap;
 }
 
+// sum over the 16 lanes of a DPP row, result in every lane of the row:
+// quad xor 1, quad xor 2, half-row mirror (quad <-> quad), row mirror (half <-> half)
+__device__ __forceinline__ float row_sum16(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, true));  // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, true));  // row_mirror
+    return v;
+}
+
+// min / max over the 16 lanes of a DPP row, result in every lane of the row
+__device__ __forceinline__ int row_min16(int v)
+{
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));
+    return v;
+}
+
+__device__ __forceinline__ int row_max16(int v)
+{
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));
+    return v;
+}
+
 // The receive window needs no per-element support test on an array of
 // n_full elements (no padding): support on every element, or the snapped
 // aperture of a window cut at t = 1 when pitch > width / 2 (elements outside a
@@ -176,6 +212,10 @@ template <int WC>
 __device__ __forceinline__ float window_eval(float t, const WinCoef &w)
 {
     if (WC == WC_RANGE) return 1.0f;
+    if (WC == WC_COS_RF) {
+        const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(fmaf(t, w.q_s, w.q_o), 0.f, 0.25f));
+        return c * c;
+    }
     float v;
     if (WC == WC_LINEAR) {
         v = w.A + w.C * t;
@@ -461,6 +501,16 @@ void beamform_kernel(BeamArgs A)
         if (A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_z);  // das_pw.m:374-377
         const Aperture ap = make_aperture(x, xp, zp, Fr, A, A.support_rx);
 
+        // elements [u_lo, u_hi] meeting the support of some voxel of this wave
+        // (wave-uniform): element steps outside carry zero window for all 16 voxels
+        int u_lo = 0, u_hi = 4 * KI - 1;
+        if (WCR != WC_COS_ALL) {
+            const bool ne = ap.klo <= ap.khi;
+            u_lo = __builtin_amdgcn_readfirstlane(row_min16(ne ? ap.klo : 0x7fff));
+            u_hi = __builtin_amdgcn_readfirstlane(row_max16(ne ? ap.khi : -1));  // -1: none
+        }
+        constexpr bool RTEST = WCR != WC_COS_ALL && WCR != WC_COS_RF;  // per-element support test
+
         floatx4 accR[MT], accI[MT];
         floatx4 tanR[TAN ? MT : 1], tanI[TAN ? MT : 1];
         float asum = 0.f;
@@ -488,7 +538,8 @@ void beamform_kernel(BeamArgs A)
             const int klo = ap.klo - g, khi = ap.khi - g;  // support in units of 4j
             const float dstep = 4.0f * pf;
             const float ul = ap.inv_wl * A.wrx.s_half, ur = ap.inv_wr * A.wrx.s_half;
-            for (int j = 0; j < KI; ++j) {
+            const int j_hi = min(u_hi >> 2, KI - 1);  // elements 4j + g, g = 0..3
+            for (int j = u_lo >> 2; j <= j_hi; ++j) {
                 const float d = fmaf((float)j, dstep, dgb);
                 float a;
                 if (WCR == WC_COS_ALL) {
@@ -498,7 +549,7 @@ void beamform_kernel(BeamArgs A)
                 } else {
                     const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
                     a = window_eval<WCR>(t, A.wrx);
-                    a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
+                    if (RTEST) a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
                 }
                 const float rv = rev_of((uint32_t)n * Qrx[j]);
                 const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
@@ -543,6 +594,7 @@ void beamform_kernel(BeamArgs A)
             }
 #pragma unroll
             for (int cc = 0; cc < KC; ++cc) {
+                if (cc < (u_lo >> 5) || cc > (u_hi >> 5)) continue;  // elements 32 cc .. 32 cc + 31
                 float br[8], bi[8];
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {
@@ -557,7 +609,7 @@ void beamform_kernel(BeamArgs A)
                     } else {
                         const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
                         a = window_eval<WCR>(t, A.wrx);
-                        a = (k >= ap.klo && k <= ap.khi) ? a : 0.f;
+                        if (RTEST) a = (k >= ap.klo && k <= ap.khi) ? a : 0.f;
                     }
                     const float rv = rev_of((uint32_t)n * Qrx[j]);
                     const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);

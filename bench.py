@@ -37,11 +37,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--arm", choices=["both", "hann", "boxcar"], default="both")
-    ap.add_argument("--workload", choices=["cpwc_full", "cpwc_small", "saft_full"], default="cpwc_full")
+    ap.add_argument("--workload", choices=["cpwc_full", "cpwc_small"], default="cpwc_full")
     ap.add_argument("--precisions", default="bf16x3,fp16x3,fp16",
                     help="mixed-precision arms of the headline configuration (comma list, '' for none)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form measurements")
     ap.add_argument("--no-batching", action="store_true", help="skip the batching-form measurements")
+    ap.add_argument("--no-saft", action="store_true", help="skip the SAFT precision-form measurements")
     ap.add_argument("--ensemble-frames", type=int, default=400)
     ap.add_argument("--queue-jobs", type=int, default=64)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -49,15 +50,38 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the beamforming kernel from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (each scaled by its reading for a device copy of known size),
+    kept in profiles/beamform_traffic.json; None when that workload was not profiled."""
+    path = os.path.join(ROOT, "profiles", "beamform_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)[workload]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def sample_column(w):
+    """lateral column of the bounded CPU sample (the centre one)"""
+    return w.positions_x.size // 2
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def cpu_baseline(w, window_name, F_kind, F_params, n_threads):
     """Reference CPU path (FP64 C restatement of das_pw.m:302-485) on a bounded sample:
-    all spectra + one lateral column of voxels, extrapolated linearly in voxels."""
+    all spectra + one lateral column of voxels, extrapolated linearly in voxels.  Also
+    returns the FP64 image of that column (CPU output scale), against which the GPU
+    arms' errors are stated."""
     from oracle import oracle as O
 
     O.build()
     rf = w.rf_host()
     F = O.fnumber(F_kind, *F_params)
-    ix_col = w.positions_x.size // 2
+    ix_col = sample_column(w)
     sel = (np.full(w.positions_z.size, ix_col), np.arange(w.positions_z.size))
     # spectra alone (the FFT of das_pw.m:383)
     N, lb, ub = O.grid(w.n_samples, w.n_elements, w.pitch, w.positions_z[0], w.f_s, w.c_0, *w.f_bounds)
@@ -65,8 +89,8 @@ def cpu_baseline(w, window_name, F_kind, F_params, n_threads):
     O.spectra(rf.reshape(-1, w.n_samples), N, lb, ub, mode=1, n_threads=n_threads)
     t_spec = time.perf_counter() - t0
     t0 = time.perf_counter()
-    O.das_pw(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0, w.f_bounds, 0,
-             (window_name, 0.0), F, "on", n_threads=n_threads, sel=sel)
+    col, _ = O.das_pw(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0, w.f_bounds, 0,
+                      (window_name, 0.0), F, "on", n_threads=n_threads, sel=sel)
     t_col = time.perf_counter() - t0 - t_spec  # das_pw recomputes the spectra
     t_frame = t_spec + max(t_col, 0.0) * w.positions_x.size
     return {
@@ -77,7 +101,88 @@ def cpu_baseline(w, window_name, F_kind, F_params, n_threads):
         "sample": f"spectra of all {w.n_tx * w.n_elements} channels + 1 of {w.positions_x.size} lateral "
                   f"columns ({w.positions_z.size} voxels, all {w.n_tx} angles), extrapolated x{w.positions_x.size}; "
                   f"sample wall {t_spec + t_col:.1f} s",
-    }
+    }, np.asarray(col).reshape(-1)
+
+
+def saft_fp64_check(w, rf, kw_oracle, n_threads, gpu_cols, n_dft):
+    """FP64 oracle (das_sa.m:289-509 restated) on the centre lateral column of the
+    SAFT workload; rel-L2 of each GPU arm's column against it, and the oracle's own
+    time as the SAFT CPU baseline (spectra + one column, extrapolated)."""
+    from oracle import oracle as O
+
+    O.build()
+    ix_col = sample_column(w)
+    sel = (np.full(w.positions_z.size, ix_col), np.arange(w.positions_z.size))
+    N, lb, ub = O.grid(w.n_samples, w.n_elements, w.pitch, w.positions_z[0], w.f_s, w.c_0, *w.f_bounds)
+    t0 = time.perf_counter()
+    O.spectra(rf.reshape(-1, w.n_samples), N, lb, ub, mode=1, n_threads=n_threads)
+    t_spec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    col, _ = O.das_sa(w.positions_x, w.positions_z, rf, w.f_s, w.width, w.pitch, w.c_0, w.f_bounds, 0.0,
+                         n_threads=n_threads, sel=sel, **kw_oracle)
+    t_col = max(time.perf_counter() - t0 - t_spec, 0.0)
+    col = np.asarray(col).reshape(-1)
+    errs = {k: rel_l2(v * n_dft, col) for k, v in gpu_cols.items()}  # GPU arms: MEX scale 2/N, oracle: 2
+    t_frame = t_spec + t_col * w.positions_x.size
+    return errs, {"value": w.n_voxels / t_frame / 1e6, "unit": "Mvoxel*frames/s", "cores": n_threads, "kind": "port",
+                  "sample": f"spectra of all {w.n_elements ** 2} channels + 1 of {w.positions_x.size} lateral columns "
+                            f"({w.positions_z.size} voxels, 128 transmits), extrapolated x{w.positions_x.size}; "
+                            f"sample wall {t_spec + t_col:.1f} s"}
+
+
+def run_saft(steps, warmup, dev, precs):
+    """SAFT precision form: full synthetic aperture 128 x 128 x 3000 samples, 512 x 512
+    voxels, Tukey(0.5) transmit and receive windows, angle_lb(60, 1.5) F-numbers on both
+    apertures, normalization on; FP32 and the mixed-precision contractions (TOF and the
+    phase reduction in FP64 in all of them).  Returns timings and the centre column of
+    every arm's image for the FP64 check."""
+    import torch
+
+    import f_number_amd as fa
+    from f_number_amd import synth
+
+    w = synth.saft_full()
+    rf = w.rf_device(dev)
+    kw = dict(window_tx=fa.windows.tukey(0.5), window_rx=fa.windows.tukey(0.5),
+              F_number_tx=fa.f_numbers.grating.angle_lb(60, 1.5), F_number_rx=fa.f_numbers.grating.angle_lb(60, 1.5),
+              normalization_tx=fa.normalizations.on(), normalization_rx=fa.normalizations.on())
+    res, cols = {}, {}
+    ix = sample_column(w)
+    ref = None
+    for p in ["fp32"] + [q for q in precs if q != "fp32"]:
+        plan = fa.SaPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.f_s, w.width, w.pitch, w.c_0,
+                         w.f_bounds, 0.0, precision=p, index_gpu=dev.index or 0, **kw)
+        info = plan.info()
+        img = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device=dev)
+        stream = torch.cuda.current_stream(dev)
+        for _ in range(warmup):
+            plan.execute(rf, img, stream=stream)
+        torch.cuda.synchronize(dev)
+        plan.set_profiling(True)
+        plan.get_profile()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.execute(rf, img, stream=stream)
+        torch.cuda.synchronize(dev)
+        wall = (time.perf_counter() - t0) / steps
+        spec_ms, beam_ms, n = plan.get_profile()
+        plan.close()
+        beam_ms /= max(n, 1)
+        r = {"ms_per_step": wall * 1e3, "beamform_ms": beam_ms, "prep_ms": spec_ms / max(n, 1),
+             "value": w.n_voxels / wall / 1e6,
+             "achieved_tflops": info["flops_per_frame"] / (beam_ms / 1e3) / 1e12,
+             "finite": bool(torch.isfinite(torch.view_as_real(img)).all().item())}
+        if p == "fp32":
+            ref = img.clone()
+        else:
+            r["rel_l2_vs_fp32"] = (torch.linalg.vector_norm(img - ref) / torch.linalg.vector_norm(ref)).item()
+            r["beamform_speedup_vs_fp32"] = res["fp32"]["beamform_ms"] / beam_ms
+        res[p] = r
+        cols[p] = img[ix].cpu().numpy().astype(np.complex128)
+        del img
+    rf_host = rf.cpu().numpy()  # the very input of the GPU arms, for the FP64 check
+    return {"workload": w.name, "n_voxels": int(w.n_voxels), "N_dft": info["N_dft"], "n_f": info["n_f"],
+            "flops_per_frame": info["flops_per_frame"], "arms": res}, w, cols, rf_host
 
 
 def run_arm(w, plan_kw, steps, warmup, dev, keep_image=False):
@@ -302,12 +407,12 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize(dev)
         del src, dst
-    w = {"cpwc_full": synth.cpwc_full, "cpwc_small": synth.cpwc_small, "saft_full": synth.saft_full}[args.workload]()
+    w = {"cpwc_full": synth.cpwc_full, "cpwc_small": synth.cpwc_small}[args.workload]()
     arms = {}
     hann_kw = dict(window=fa.windows.hann(), F_number=fa.f_numbers.grating.angle_lb(60, 1.5))
     precs = [p for p in args.precisions.split(",") if p]
     if args.arm in ("both", "hann"):
-        arms["hann_angle_lb"] = run_arm(w, hann_kw, args.steps, args.warmup, dev, keep_image=bool(precs))
+        arms["hann_angle_lb"] = run_arm(w, hann_kw, args.steps, args.warmup, dev, keep_image=True)
     if args.arm in ("both", "boxcar"):
         arms["boxcar_F1.5"] = run_arm(w, dict(window=fa.windows.boxcar(), F_number=fa.f_numbers.constant(1.5)),
                                       args.steps, args.warmup, dev)
@@ -316,8 +421,11 @@ def main():
     # mixed-precision contractions of the headline configuration: speed-up of
     # the beamforming kernel and image error relative to the FP32 image
     prec_arms = {}
+    ix_col = sample_column(w)
+    cols = {}  # centre lateral column of each arm's image, for the FP64 check of the CPU leg
     if head_key == "hann_angle_lb":
         ref_img = head["image"]
+        cols["fp32"] = ref_img[ix_col].cpu().numpy().astype(np.complex128)
         for p in precs:
             r = run_arm(w, dict(hann_kw, precision=p), args.steps, args.warmup, dev, keep_image=True)
             err = (torch.linalg.vector_norm(r["image"] - ref_img) / torch.linalg.vector_norm(ref_img)).item()
@@ -325,6 +433,7 @@ def main():
                             "prep_ms": r["spectra_ms"], "value": w.n_voxels / (r["ms_per_step"] / 1e3) / 1e6,
                             "beamform_speedup_vs_fp32": head["beamform_ms"] / r["beamform_ms"],
                             "rel_l2_vs_fp32": err, "finite": r["finite"]}
+            cols[p] = r["image"][ix_col].cpu().numpy().astype(np.complex128)
             del r
     value = w.n_voxels / (head["ms_per_step"] / 1e3) / 1e6
     flops = head["info"]["flops_per_frame"]
@@ -358,7 +467,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(w.name),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "flops_per_launch": flops,
             "avg_launch_ms": head["beamform_ms"],
         },
@@ -372,9 +482,31 @@ def main():
         out["differentiation"] = run_diff(max(3, args.steps // 2), 1, dev)
     if not args.no_batching:
         out["batching"] = run_batching(max(2, args.steps // 3), args.ensemble_frames, args.queue_jobs, dev)
+    saft = None
+    if not args.no_saft:
+        saft, w_sa, cols_sa, rf_sa = run_saft(max(2, args.steps // 2), 1, dev, precs)
+        out["saft"] = saft
     if not args.no_cpu_baseline:
         nthreads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(w, "hann", "angle_lb", (60, 1.5, 10), nthreads)
+        if head_key == "hann_angle_lb":
+            cb, ref_col = cpu_baseline(w, "hann", "angle_lb", (60, 1.5, 10), nthreads)
+            out["cpu_baseline"] = cb
+            # error of every arm against the FP64 oracle on the sampled column (GPU arms use
+            # the MEX output scale 2/N; the oracle the CPU scale 2)
+            n_dft = head["info"]["N_dft"]
+            errs = {k: rel_l2(v * n_dft, ref_col) for k, v in cols.items()}
+            out["fp64_check"] = {"sample": f"lateral column {ix_col} ({w.positions_z.size} voxels)", "rel_l2": errs}
+            for k, e in errs.items():
+                if k in prec_arms:
+                    prec_arms[k]["rel_l2_vs_fp64"] = e
+        if saft is not None:
+            errs, cb_sa = saft_fp64_check(
+                w_sa, rf_sa, dict(window_tx=("tukey", 0.5), window_rx=("tukey", 0.5)), nthreads, cols_sa,
+                saft["N_dft"])
+            for k, e in errs.items():
+                saft["arms"][k]["rel_l2_vs_fp64"] = e
+            saft["fp64_check_sample"] = f"lateral column {sample_column(w_sa)} ({w_sa.positions_z.size} voxels)"
+            saft["cpu_baseline"] = cb_sa
     print(json.dumps(out), flush=True)
 
 

@@ -130,6 +130,7 @@ PROTOTYPES = {
                                      C.POINTER(C.c_void_p), C.c_void_p]),
     "fn_batch_info": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "fn_batch_destroy": (C.c_int32, [C.c_void_p]),
+    "fn_plan_focused_signal": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fn_plan_set_tof": (C.c_int32, [C.c_void_p, C.c_double, _dp]),
     "fn_plan_execute_tangent": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),

@@ -34,6 +34,8 @@ def _units(extra):
         ("fn_core.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_core.hip")]),
         ("fn_synth.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_synth.hip")]),
         ("fn_adjoint.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_adjoint.hip")]),
+        ("fn_windowed.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_windowed.hip")]),
+        ("fn_signal.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_signal.hip")]),
     ]
     for mode in (0, 1):
         for ki in (8, 16, 32):

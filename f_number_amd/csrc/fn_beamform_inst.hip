@@ -169,4 +169,49 @@ hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, 
 }
 #endif
 
+#if INST_PREC == 0
+// windowed-normalization variant (FP32, general window classes): plane-wave
+// groups of at most two transmit tiles, synthetic aperture at its fixed tiling
+#define FN_NW_LAUNCHER(m, k) FN_CAT6(launch_beamform_nw_m, m, _k, k, , )
+template <int MT, int WCR, int WCT>
+static hipError_t launch_nw_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    auto kern = beamform_kernel<INST_MODE, MT, INST_KI, WCR, WCT, 0, false, false, true>;
+    static size_t lds_done = 0;
+    if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
+    return hipGetLastError();
+}
+
+template <int MT, int WCT>
+static hipError_t launch_nw_wcr(int wcr, dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
+{
+    switch (wcr) {
+    case WC_RANGE: return launch_nw_one<MT, WC_RANGE, WCT>(grid, lds, s, A);
+    case WC_LINEAR: return launch_nw_one<MT, WC_LINEAR, WCT>(grid, lds, s, A);
+    case WC_COS: return launch_nw_one<MT, WC_COS, WCT>(grid, lds, s, A);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t FN_NW_LAUNCHER(INST_MODE, INST_KI)(int mt, int wcr, int wct, dim3 grid, size_t lds, hipStream_t s,
+                                             const BeamArgs &A)
+{
+#if INST_MODE == 0
+    (void)wct;
+    if (mt == 1) return launch_nw_wcr<1, WC_RANGE>(wcr, grid, lds, s, A);
+    if (mt == 2) return launch_nw_wcr<2, WC_RANGE>(wcr, grid, lds, s, A);
+    return hipErrorInvalidValue;
+#else
+    if (mt != INST_KI / 4) return hipErrorInvalidValue;
+    switch (wct) {
+    case WC_RANGE: return launch_nw_wcr<INST_KI / 4, WC_RANGE>(wcr, grid, lds, s, A);
+    case WC_LINEAR: return launch_nw_wcr<INST_KI / 4, WC_LINEAR>(wcr, grid, lds, s, A);
+    case WC_COS: return launch_nw_wcr<INST_KI / 4, WC_COS>(wcr, grid, lds, s, A);
+    default: return hipErrorInvalidValue;
+    }
+#endif
+}
+#endif
+
 }  // namespace fnk

@@ -43,6 +43,16 @@ static const EnsLauncher kEnsLaunchers[3][4] = {
     {launch_beamform_ens_k8_p0, launch_beamform_ens_k8_p1, launch_beamform_ens_k8_p2, launch_beamform_ens_k8_p3},
     {launch_beamform_ens_k16_p0, launch_beamform_ens_k16_p1, launch_beamform_ens_k16_p2, launch_beamform_ens_k16_p3},
     {launch_beamform_ens_k32_p0, launch_beamform_ens_k32_p1, launch_beamform_ens_k32_p2, launch_beamform_ens_k32_p3}};
+hipError_t launch_beamform_nw_m0_k8(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_nw_m0_k16(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_nw_m0_k32(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_nw_m1_k8(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_nw_m1_k16(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_nw_m1_k32(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_windowed_rows(const BeamArgs &A, int mode, int win_rx, double par_rx, float *h_rx, int win_tx,
+                                double par_tx, float *h_tx, hipStream_t s);
+hipError_t launch_focused_signal(const BeamArgs &A, int mode, int wcr, int wct, int MT_all, int KI, int nw, int ix,
+                                 int iz, float2 *F, const float2 *tw, float2 *out, hipStream_t s);
 hipError_t launch_beamform_tan_k8(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k16(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k32(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
@@ -56,6 +66,9 @@ hipError_t launch_vjp_delay(const float2 *X, const float2 *Z, double *out, int n
 hipError_t launch_dot_re(const float2 *g, const float2 *d, long n, double *part, int n_part, double *out, hipStream_t s);
 typedef hipError_t (*BeamLauncher)(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 // [mode][KI = 8, 16, 32][precision]
+static const BeamLauncher kNwLaunchers[2][3] = {
+    {launch_beamform_nw_m0_k8, launch_beamform_nw_m0_k16, launch_beamform_nw_m0_k32},
+    {launch_beamform_nw_m1_k8, launch_beamform_nw_m1_k16, launch_beamform_nw_m1_k32}};
 static const BeamLauncher kBeamLaunchers[2][3][4] = {
     {{launch_beamform_m0_k8_p0, launch_beamform_m0_k8_p1, launch_beamform_m0_k8_p2, launch_beamform_m0_k8_p3},
      {launch_beamform_m0_k16_p0, launch_beamform_m0_k16_p1, launch_beamform_m0_k16_p2, launch_beamform_m0_k16_p3},
@@ -93,6 +106,11 @@ struct fn_plan {
     int KI = 0, MT_all = 0, MT_tmpl = 0, n_groups = 1, bin_split = 1;
     int n_frames = 1;  // ensemble frames (transmit rows = frames x angles)
     int wcr = 0, wct = 0;
+    int nw = 0;                 // windowed normalization (CPU semantics): taper kernels, one bin chunk
+    float *d_nw = nullptr;      // L / 2 per (voxel, element): receive, then (SA) transmit
+    int executed = 0;           // spectra of an execute are in d_X (focused signal)
+    float2 *d_sigF = nullptr;   // focused spectrum of the signal voxel [n_f]
+    std::vector<double> h_px, h_pz, h_Frx, h_Ftx, h_sqf;  // host copies (signal voxel search)
     int precision = FN_PRECISION_FP32;
     int verbosity = 0;
     fnk::BeamArgs args{};
@@ -189,6 +207,48 @@ int setup_window(const fn_window &w, int semantics, fnk::WinCoef *c, int *wc, in
     }
 }
 
+// taper of the windowed normalization in CPU semantics as window_any()
+// coefficients (|t|: flat part, A + C t + B cos(2 pi (t - t_off) s_half), cut)
+int taper_coef(const fn_window &w, fnk::WinCoef *c)
+{
+    *c = fnk::WinCoef{-1.f, INFINITY, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    switch (w.id) {
+    case FN_WINDOW_BOXCAR: c->t_cut = 1.f; return FN_OK;                       // |t| < 1
+    case FN_WINDOW_HANN: c->A = c->B = 0.5f; c->s_half = 0.5f; return FN_OK;   // untruncated
+    case FN_WINDOW_TUKEY:
+        c->t_flat = c->t_off = (float)(1.0 - w.parameter);
+        c->t_cut = 1.f;
+        c->A = c->B = 0.5f;
+        c->s_half = (float)(0.5 / w.parameter);
+        return FN_OK;
+    case FN_WINDOW_TRIANGULAR: c->A = 1.f; c->C = -1.f; return FN_OK;          // 1 - |t|, untruncated
+    default: return fail(FN_ERR_WINDOW, "window id %d is not reachable from any windows.* class", (int)w.id);
+    }
+}
+
+// windowed normalization: L / 2 per (voxel, element) from the FP64 row pass,
+// once per plan (geometry and F-number tables only)
+int setup_windowed(fn_plan *P, bool rx, const fn_window &wrx, const fn_window &nwin_rx, bool tx, const fn_window &wtx,
+                   const fn_window &nwin_tx)
+{
+    if (P->precision != FN_PRECISION_FP32)
+        return fail(FN_ERR_UNSUPPORTED, "normalizations.windowed runs in FP32 only");
+    if (P->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "normalizations.windowed: single-frame plans only");
+    const size_t n = (size_t)P->n_vox * P->n_el;
+    HIP_CHECK(hipMalloc(&P->d_nw, n * (rx + tx) * sizeof(float)), "windowed normalization rows");
+    float *h_rx = rx ? P->d_nw : nullptr, *h_tx = tx ? P->d_nw + (rx ? n : 0) : nullptr;
+    int s;
+    if (rx && (s = taper_coef(nwin_rx, &P->args.wn_rx))) return s;
+    if (tx && (s = taper_coef(nwin_tx, &P->args.wn_tx))) return s;
+    hipError_t e = fnk::launch_windowed_rows(P->args, P->mode, wrx.id, wrx.parameter, h_rx, wtx.id, wtx.parameter, h_tx,
+                                             P->stream);
+    if (e != hipSuccess) return hip_fail(e, "windowed_rows_kernel");
+    HIP_CHECK(hipStreamSynchronize(P->stream), "windowed_rows_kernel");
+    P->args.nw_rx = h_rx;
+    P->args.nw_tx = h_tx;
+    return FN_OK;
+}
+
 // forward element loops without per-element support tests: the raised cosine
 // over every element of a full array (CPU Hann) takes WC_COS_ALL, a cosine
 // window whose cut implies the support takes WC_COS_RF; tangent and adjoint
@@ -196,7 +256,7 @@ int setup_window(const fn_window &w, int semantics, fnk::WinCoef *c, int *wc, in
 void use_cos_all(fn_plan *P)
 {
     const fnk::WinCoef &w = P->args.wrx;
-    if (P->wcr != fnk::WC_COS) return;
+    if (P->wcr != fnk::WC_COS || P->nw) return;
     if (P->args.support_rx == fnk::SUPPORT_ALL && P->n_el == 4 * P->KI && w.t_off == 0.f && w.t_flat < 0.f &&
         std::isinf(w.t_cut))
         P->wcr = fnk::WC_COS_ALL;
@@ -237,7 +297,8 @@ void destroy(fn_plan *p)
     if (!p) return;
     hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp); hipFree(p->d_sqf);
     hipFree(p->d_tw); hipFree(p->d_X); hipFree(p->d_Xlp); hipFree(p->d_xscale);
-    hipFree(p->d_delay); hipFree(p->d_P); hipFree(p->d_Z); hipFree(p->d_part); hipFree(p->d_qtab);
+    hipFree(p->d_delay); hipFree(p->d_P); hipFree(p->d_Z); hipFree(p->d_part); hipFree(p->d_qtab); hipFree(p->d_nw);
+    hipFree(p->d_sigF);
     for (auto &r : p->pending) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     for (auto &r : p->spare) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     if (p->stream) hipStreamDestroy(p->stream);
@@ -271,7 +332,7 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     if (P->mode == 0) {
         // at most 5 transmit tiles per group keep the LDS tile within 80 KiB
         // (ensembles: 4, for the per-row image accumulators)
-        const int max_tiles = tile_cap(P->n_frames, c.precision);
+        const int max_tiles = P->nw ? 2 : tile_cap(P->n_frames, c.precision);
         const int groups = (P->MT_all + max_tiles - 1) / max_tiles;
         P->MT_tmpl = (P->MT_all + groups - 1) / groups;
         P->n_groups = (P->MT_all + P->MT_tmpl - 1) / P->MT_tmpl;
@@ -291,8 +352,8 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     // bin split: enough workgroups to fill 256 CUs several times over
     const long tiles = (P->n_vox + 63) / 64;
     const long blocks = tiles * P->n_groups;
-    int split = 1;
-    if (blocks < 2048) split = (int)std::min<long>((2048 + blocks - 1) / blocks, std::max(1, g.n_f / 8));
+    int split = 1;  // windowed normalization: every block walks all bins (valid-row counts)
+    if (blocks < 2048 && !P->nw) split = (int)std::min<long>((2048 + blocks - 1) / blocks, std::max(1, g.n_f / 8));
     const int chunk = (g.n_f + split - 1) / split;
     P->bin_split = (g.n_f + chunk - 1) / chunk;
 
@@ -305,6 +366,11 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     std::vector<double> sqf(g.n_f);
     for (int b = 0; b < g.n_f; ++b) sqf[b] = std::sqrt(c.f_s * (double)(g.n_lb + b) / g.N);
     if ((s = upload(&P->d_sqf, sqf.data(), sqf.size(), "sqrt(f)"))) return s;
+    P->h_px.assign(c.px, c.px + c.n_x);
+    P->h_pz.assign(c.pz, c.pz + c.n_z);
+    P->h_Frx.assign(Frx, Frx + g.n_f);
+    if (Ftx) P->h_Ftx.assign(Ftx, Ftx + g.n_f);
+    P->h_sqf = sqf;
     // twiddles e^{+j 2 pi m / N} rounded once from long double (conjugated in the kernel)
     std::vector<float2> tw(g.N);
     for (int m = 0; m < g.N; ++m) {
@@ -375,6 +441,7 @@ double useful_fraction(const fnh::Grid &g, const double *px, int nx, const doubl
 hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipStream_t s)
 {
     const int kidx = P->KI == 8 ? 0 : (P->KI == 16 ? 1 : 2);
+    if (P->nw) return fnk::kNwLaunchers[P->mode][kidx](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
     if (P->n_frames > 1) return fnk::kEnsLaunchers[kidx][P->precision](P->MT_tmpl, P->wcr, grid, P->lds_beam, s, A);
     return fnk::kBeamLaunchers[P->mode][kidx][P->precision](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
 }
@@ -391,6 +458,7 @@ hipError_t lds_limit(const void *kern, size_t lds, size_t &done)
 
 int launch_spectra(fn_plan *P, const void *d_rf, int dtype, hipStream_t s)
 {
+    P->executed = 1;
     const fnk::SpecArgs &S = P->sargs;
     dim3 sg((unsigned)((S.n_f + 63) / 64), (unsigned)((S.n_ch + 63) / 64));
     static size_t done_f = 0, done_d = 0;
@@ -497,12 +565,13 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
     fnh::Grid g;
     int s = fnh::validate_pw(p, &g);
     if (s) return s;
-    if (p->normalization == FN_NORMALIZATION_WINDOWED)
-        return fail(FN_ERR_UNSUPPORTED, "normalizations.windowed is not available in this build");
     fn_plan *P = new (std::nothrow) fn_plan();
     if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
     P->mode = 0;
     P->n_frames = std::max(1, p->n_frames);
+    // windowed normalization: the CPU path tapers (windowed.m:81-86); the
+    // reference MEX reads the window and normalizes as "on" (gpu_bf_das_pw_rf.cu:421-427)
+    P->nw = p->normalization == FN_NORMALIZATION_WINDOWED && p->semantics == FN_SEMANTICS_CPU;
     int wsup = 0;
     if ((s = setup_window(p->window, p->semantics, &P->args.wrx, &P->wcr, &wsup))) { destroy(P); return s; }
     P->args.support_rx = wsup;
@@ -528,6 +597,8 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
     }
     if ((s = upload(&P->d_txp, txp.data(), txp.size(), "steering"))) { destroy(P); return s; }
     P->args.txp = P->d_txp;
+    if (P->nw && (s = setup_windowed(P, true, p->window, p->normalization_window, false, p->window,
+                                     p->normalization_window))) { destroy(P); return s; }
     use_cos_all(P);
     P->useful_flops = P->flops * useful_fraction(g, p->positions_x, p->n_x, p->positions_z, p->n_z, p->f_number_values,
                                                  p->f_s, p->c_0, p->element_pitch, p->n_elements, P->args.clamp != 0);
@@ -540,11 +611,13 @@ int sa_plan(const fn_sa_params *p, fn_plan **out)
     fnh::Grid g;
     int s = fnh::validate_sa(p, &g);
     if (s) return s;
-    if (p->normalization_tx == FN_NORMALIZATION_WINDOWED || p->normalization_rx == FN_NORMALIZATION_WINDOWED)
-        return fail(FN_ERR_UNSUPPORTED, "normalizations.windowed is not available in this build");
     fn_plan *P = new (std::nothrow) fn_plan();
     if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
     P->mode = 1;
+    const bool cpu = p->semantics == FN_SEMANTICS_CPU;
+    const bool nw_rx = cpu && p->normalization_rx == FN_NORMALIZATION_WINDOWED;
+    const bool nw_tx = cpu && p->normalization_tx == FN_NORMALIZATION_WINDOWED;
+    P->nw = nw_rx || nw_tx;
     int sup = 0;
     if ((s = setup_window(p->window_rx, p->semantics, &P->args.wrx, &P->wcr, &sup))) { destroy(P); return s; }
     P->args.support_rx = sup;
@@ -560,6 +633,8 @@ int sa_plan(const fn_sa_params *p, fn_plan **out)
     P->useful_flops = P->flops * useful_fraction(g, p->positions_x, p->n_x, p->positions_z, p->n_z,
                                                  p->f_number_rx_values, p->f_s, p->c_0, p->element_pitch,
                                                  p->n_elements, false);
+    if (P->nw && (s = setup_windowed(P, nw_rx, p->window_rx, p->normalization_window_rx, nw_tx, p->window_tx,
+                                     p->normalization_window_tx))) { destroy(P); return s; }
     use_cos_all(P);
     *out = P;
     return FN_OK;
@@ -572,7 +647,7 @@ size_t dtype_size(int dtype) { return dtype == FN_DTYPE_F64 ? 8 : 4; }
 int max_tiles(const fn_plan *P)
 {
     if (P->mode == 1) return P->KI / 4;
-    return std::min(P->MT_all, tile_cap(P->n_frames, P->precision));
+    return std::min(P->MT_all, P->nw ? 2 : tile_cap(P->n_frames, P->precision));
 }
 
 int set_tiling(fn_plan *P, int mt, int split)
@@ -585,6 +660,7 @@ int set_tiling(fn_plan *P, int mt, int split)
     }
     const int n_f = P->g.n_f;
     if (split < 1 || split > n_f) return fail(FN_ERR_ARGUMENT, "bin_split must be in 1..%d", n_f);
+    if (P->nw && split != 1) return fail(FN_ERR_ARGUMENT, "windowed normalization runs with bin_split 1");
     const int planes = P->precision == FN_PRECISION_FP16 ? 2 : 4;
     const size_t step_bytes = P->precision == FN_PRECISION_FP32 ? 64 * sizeof(float2) : (size_t)planes * 64 * 2;
     const size_t lds = (size_t)mt * P->KI * step_bytes;
@@ -615,7 +691,8 @@ int autotune(fn_plan *P, const void *d_rf, int dtype, float *d_image, int reps, 
             if ((P->MT_all + groups - 1) / groups != mt) continue;
         }
         for (int split : {1, 2, 4, 8, 16, split0})
-            if (split <= n_f && std::find(cand.begin(), cand.end(), std::make_pair(mt, split)) == cand.end())
+            if (split <= n_f && (!P->nw || split == 1) &&
+                std::find(cand.begin(), cand.end(), std::make_pair(mt, split)) == cand.end())
                 cand.emplace_back(mt, split);
     }
     hipEvent_t e0, e1;
@@ -680,6 +757,7 @@ int check_pw_fp32(const fn_plan *P, const char *what)
     if (P->mode != 0) return fail(FN_ERR_UNSUPPORTED, "%s: plane-wave plans only", what);
     if (P->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "%s: single-frame plans only", what);
     if (P->n_el > 128) return fail(FN_ERR_UNSUPPORTED, "%s: at most 128 elements", what);
+    if (P->nw) return fail(FN_ERR_UNSUPPORTED, "%s: normalizations.windowed is not differentiated", what);
     return FN_OK;
 }
 
@@ -799,6 +877,50 @@ int vjp(fn_plan *P, const void *d_rf, int dtype, const float *d_g, const float *
     return FN_OK;
 }
 
+// a bin of voxel (ix, iz) with non-empty aperture(s): the FP64 bounds of
+// make_aperture (das_pw.m:399-414, das_sa.m:384-414)
+bool bin_valid(const fn_plan *P, int ix, int iz, int b)
+{
+    const fnk::BeamArgs &A = P->args;
+    const double x = P->h_px[ix], z = P->h_pz[iz];
+    const double xp = x / A.pitch, zp = z / (2.0 * A.pitch);
+    auto ok = [&](double F) {
+        const double hp = zp / F;
+        const double ua = A.M + (xp - hp), ub = A.M + (xp + hp);
+        return std::fmax(std::ceil(ua), 0.0) <= std::fmin(std::floor(ub), (double)(A.n_el - 1));
+    };
+    double Fr = P->h_Frx[b];
+    if (A.clamp) Fr = std::fmin(Fr, P->h_sqf[b] * std::sqrt(z / (2.88 * A.c0)));  // das_pw.m:374-377
+    bool v = ok(Fr);
+    if (P->mode == 1) v = v && ok(P->h_Ftx[b]);
+    return v;
+}
+
+// focused RF signal of the last voxel the CPU loop processes (das_pw.m:477-481,
+// das_sa.m:501-505) from the spectra of the plan's last execute: complex [N_dft]
+int focused_signal(fn_plan *P, float *d_signal, hipStream_t s)
+{
+    if (!P || !d_signal) return fail(FN_ERR_ARGUMENT, "null argument");
+    if (P->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "focused signal: single-frame plans only");
+    if (!P->executed) return fail(FN_ERR_ARGUMENT, "focused signal: execute the plan first");
+    int vx = -1, vz = -1;  // last lateral position with a valid bin, its last such depth
+    for (int ix = P->n_x - 1; ix >= 0 && vx < 0; --ix)
+        for (int iz = P->n_z - 1; iz >= 0 && vx < 0; --iz)
+            for (int b = 0; b < P->g.n_f; ++b)
+                if (bin_valid(P, ix, iz, b)) {
+                    vx = ix;
+                    vz = iz;
+                    break;
+                }
+    if (vx < 0) return fail(FN_ERR_ARGUMENT, "focused signal: no voxel has a non-empty aperture");
+    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
+    if (!P->d_sigF) HIP_CHECK(hipMalloc(&P->d_sigF, (size_t)P->g.n_f * sizeof(float2)), "signal spectrum");
+    hipError_t e = fnk::launch_focused_signal(P->args, P->mode, P->wcr, P->wct, P->MT_all, P->KI, P->nw, vx, vz,
+                                              P->d_sigF, P->d_tw, reinterpret_cast<float2 *>(d_signal), s);
+    if (e != hipSuccess) return hip_fail(e, "focused_spectrum_kernel");
+    return FN_OK;
+}
+
 // one-shot: H2D, execute, D2H (gpu_bf_das_pw_rf.cu:730-1122)
 int one_shot(fn_plan *P, const void *rf, int dtype, float *image, float *w0, float *w1)
 {
@@ -903,6 +1025,12 @@ int32_t fn_plan_execute_tangent(fn_plan *P, const void *d_rf, int32_t dtype, flo
 {
     fnh::clear_error();
     return execute_tangent(P, d_rf, dtype, d_image, d_dimage_dc, d_w, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t fn_plan_focused_signal(fn_plan *P, float *d_signal, void *stream)
+{
+    fnh::clear_error();
+    return focused_signal(P, d_signal, reinterpret_cast<hipStream_t>(stream));
 }
 
 int32_t fn_plan_adjoint(fn_plan *P, const float *d_y, float *d_rf_adj, void *stream)

@@ -90,6 +90,10 @@ struct BeamArgs {
     float *dimage;          // tangent: dI / dc_tof, interleaved complex [n_vox]
     WinCoef wrx, wtx;
     float out_scale;
+    // windowed normalization (windowed.m:81-86): half the number of positive rows
+    // L / 2 per (voxel, element), receive and (SA) transmit, and the taper windows
+    const float *nw_rx, *nw_tx;  // [n_vox][n_el]; 0 where L = 0
+    WinCoef wn_rx, wn_tx;
 };
 
 // adjoint voxel contraction (fn_adjoint.hip)
@@ -227,6 +231,26 @@ __device__ __forceinline__ float window_eval(float t, const WinCoef &w)
     v = (t <= w.t_flat) ? 1.0f : v;
     v = (t < w.t_cut) ? v : 0.0f;
     return v;
+}
+
+// every window form of WinCoef at once (|t| of the argument): flat part, raised
+// cosine plus linear term, cut -- the taper of the windowed normalization,
+// evaluated in CPU semantics (windows/*.m compute_samples_scalar)
+__device__ __forceinline__ float window_any(float t, const WinCoef &w)
+{
+    const float ta = fabsf(t);
+    float u = (ta - w.t_off) * w.s_half;
+    u = u - rintf(u);
+    float v = fmaf(w.C, ta, w.A) + w.B * __builtin_amdgcn_cosf(u);
+    v = (ta <= w.t_flat) ? 1.0f : v;
+    return (ta < w.t_cut) ? v : 0.0f;
+}
+
+// taper factor of a row: window((i - L/2) / (L/2)); columns without positive
+// rows (h = 0) keep their zero weights
+__device__ __forceinline__ float taper_at(int row, float h, const WinCoef &w)
+{
+    return h > 0.f ? window_any(((float)row - h) / h, w) : 0.f;
 }
 
 // -------------------------------------------------------------------------
@@ -390,12 +414,15 @@ struct WavesPerEU {
 // ENS (plane waves): the transmit rows are (frame, angle) pairs of an ensemble
 // sharing one geometry; every row keeps its own image accumulator over the
 // bins and is added to its frame's image at the end (frame = row / rows_per_frame).
-template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false>
+// NW: windowed normalization (FP32, one bin chunk per block so that every lane
+// counts its voxel's valid rows from the first bin).
+template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<MT, KI>::value)))
 void beamform_kernel(BeamArgs A)
 {
     static_assert(!TAN || (MODE == MODE_PW && PREC == PREC_FP32), "tangent: plane waves, FP32");
     static_assert(!ENS || (MODE == MODE_PW && !TAN), "ensemble: plane waves");
+    static_assert(!NW || (PREC == PREC_FP32 && !TAN && !ENS), "windowed normalization: FP32 forward");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, c = lane & 15;
@@ -454,6 +481,7 @@ void beamform_kernel(BeamArgs A)
     const float dg_tx = (float)((4.0 * g - A.M) * A.pitch - x);  // SA transmit rows, mt = r = 0
 
     float Ir = 0.f, Ii = 0.f, w_last = 0.f, wtx_last = 0.f;
+    int nw_row = 0;  // windowed normalization: valid rows of this voxel before the current bin
     float Dr = 0.f, Di = 0.f;  // tangent accumulators
     float Er[ENS ? MT : 1][4], Ei[ENS ? MT : 1][4];  // ensemble: per-row image accumulators
     if (ENS) {
@@ -553,9 +581,13 @@ void beamform_kernel(BeamArgs A)
                 }
                 const float rv = rev_of((uint32_t)n * Qrx[j]);
                 const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                asum += a;  // the normalizing sum is taken before the taper (on, then windowed)
+                if (NW && A.nw_rx) {  // null when only the transmit side is windowed (SA)
+                    const int k = 4 * j + g;
+                    a *= k < A.n_el ? taper_at(nw_row, A.nw_rx[vv * A.n_el + k], A.wn_rx) : 0.f;
+                }
                 const float br = a * cs, bi = a * sn;
                 const float bd = bi - br, bp = br + bi;
-                asum += a;
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
                     const float2 xv = xs[(mt * KI + j) * 64 + lane];
@@ -684,6 +716,7 @@ void beamform_kernel(BeamArgs A)
                     Di = fmaf(f, pr, Di);
                 }
             }
+            if (NW) nw_row += ap.valid ? 1 : 0;
         } else {
             // transmit aperture and apodization (das_sa.m:384-400, 438-455)
             const double Ft = A.Ftx[b];
@@ -704,6 +737,15 @@ void beamform_kernel(BeamArgs A)
                 }
             tsum += __shfl_xor(tsum, 16);
             tsum += __shfl_xor(tsum, 32);
+            if (NW && A.nw_tx) {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = 16 * (mt0 + mt) + 4 * g + r;
+                        wtx[mt][r] *= i < A.n_el ? taper_at(nw_row, A.nw_tx[vv * A.n_el + i], A.wn_tx) : 0.f;
+                    }
+            }
             if (ap.valid && at.valid) {
                 const float sc = (A.norm_rx ? 1.0f / asum : 1.0f) * (A.norm_tx ? 1.0f / tsum : 1.0f) * xscale;
                 float pr = 0.f, pi = 0.f;
@@ -720,6 +762,7 @@ void beamform_kernel(BeamArgs A)
                 Ir = fmaf(sc, pr, Ir);
                 Ii = fmaf(sc, pi, Ii);
             }
+            if (NW) nw_row += (ap.valid && at.valid) ? 1 : 0;
             if (b == A.n_f - 1) wtx_last = tsum;
         }
         if (b == A.n_f - 1) w_last = asum;

@@ -194,7 +194,7 @@ def sa_params(px, pz, n_t, n_el, f_s, w, p, c, fb, t0, window_tx, window_rx, F_t
 def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width, element_pitch, c_0,
            f_bounds=None, index_t0=None, window=None, F_number=None, normalization=None, platform=None, *,
            semantics: str = "cpu", precision: str = "fp32", output_scale: str = "mex",
-           return_weights: bool = False, verbosity: int = 0):
+           return_weights: bool = False, verbosity: int = 0, nargout: int = 1):
     """Fourier-domain DAS of steered plane waves (das_pw.m:1).
 
     Returns the complex64 image [N_z, N_x] in the MEX scale 2/N_dft * sum
@@ -203,6 +203,10 @@ def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width
     [N_t, N_el, N_a]: the result is the compounded image.  With
     return_weights=True also returns the summed receive apodization weights of
     the last frequency (the MEX's second output) and an info dict.
+    nargout=2 or 3 returns (image, F_number_values[, signal]) like the CPU path
+    (das_pw.m:1): the F-numbers per bin and depth [N_f, N_z] (clamped at
+    sqrt(f z / 2.88 c) in CPU semantics, das_pw.m:374-377) and the focused RF
+    signal of the last voxel [N_dft] (das_pw.m:477-481; angles summed).
     """
     prefix = "das_pw"
     px, pz, rf, f_s, w, p, c, fb, t0 = _common_checks(prefix, positions_x, positions_z, data_RF, f_s,
@@ -223,6 +227,19 @@ def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width
     n_t, n_el = rf.shape[0], rf.shape[1]
     P, keep, info = pw_params(px, pz, n_t, n_el, ang, f_s, w, p, c, fb, t0, win, F, nrm, pf.index,
                               semantics, precision, output_scale, verbosity)
+    if nargout >= 2:
+        f = f_s * np.arange(info["n_lb"], info["n_ub"] + 1) / info["N_dft"]
+        Fz = np.repeat(info["F_number_values"][:, None], pz.size, axis=1)
+        if semantics == "cpu":
+            Fz = np.minimum(Fz, np.sqrt(f[:, None] * pz[None, :] / (2.88 * c)))
+        outs = (Fz,)
+        if nargout >= 3:
+            from .plan import PwPlan
+
+            plan = PwPlan(px, pz, n_t, n_el, ang, f_s, w, p, c, fb, t0, window=win, F_number=F, normalization=nrm,
+                          index_gpu=pf.index, semantics=semantics, precision=precision, output_scale=output_scale)
+            image, sig = _execute_with_signal(plan, rf, px.size, pz.size, pf.index)
+            return (image,) + outs + (sig,)
     rfF = np.asfortranarray(rf, dtype=np.float64 if rf.dtype == np.float64 else np.float32)
     dtype = _lib.DTYPE["float64" if rfF.dtype == np.float64 else "float32"]
     img = np.empty(2 * px.size * pz.size, dtype=np.float32)
@@ -232,15 +249,36 @@ def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width
                            wts.ctypes.data_as(_fp) if wts is not None else None), "gpu_bf_das_pw_rf")
     del keep
     image = _image_out(img, px.size, pz.size)
+    if nargout >= 2:
+        return (image,) + outs
     if return_weights:
         return image, wts.reshape(px.size, pz.size).T, info
     return image
 
 
+def _execute_with_signal(plan, rf, n_x, n_z, index_gpu):
+    """image [N_z, N_x] and focused signal [N_dft] through a plan (device path)"""
+    import torch
+
+    dev = torch.device("cuda", index_gpu)
+    # MATLAB [N_t, N_el, N_tx] -> device (N_tx, N_el, N_t)
+    host = np.ascontiguousarray(np.transpose(rf if rf.ndim == 3 else rf[:, :, None], (2, 1, 0)))
+    x = torch.from_numpy(host.astype(np.float64 if rf.dtype == np.float64 else np.float32)).to(dev)
+    img = torch.empty((n_x, n_z), dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    plan.execute(x, img, stream=stream)
+    sig = plan.focused_signal(stream=stream, device=dev)
+    torch.cuda.synchronize(dev)
+    out = img.cpu().numpy().T.copy(), sig.cpu().numpy()
+    plan.close()
+    return out
+
+
 def das_sa(positions_x, positions_z, data_RF, f_s, element_width, element_pitch, c_0, f_bounds=None,
            index_t0=None, window_tx=None, window_rx=None, F_number_tx=None, F_number_rx=None,
            normalization_tx=None, normalization_rx=None, platform=None, *, semantics: str = "cpu",
-           precision: str = "fp32", output_scale: str = "mex", return_weights: bool = False, verbosity: int = 0):
+           precision: str = "fp32", output_scale: str = "mex", return_weights: bool = False, verbosity: int = 0,
+           nargout: int = 1):
     """Fourier-domain DAS of a full synthetic-aperture dataset (das_sa.m:1).
 
     data_RF is [N_t, N_rx, N_tx] with N_rx == N_tx.  The receive window,
@@ -248,6 +286,8 @@ def das_sa(positions_x, positions_z, data_RF, f_s, element_width, element_pitch,
     complex64 image [N_z, N_x] (MEX scale unless output_scale='cpu'); with
     return_weights=True also the transmit and receive weight sums of the last
     frequency (gpu_bf_saft_rf.cu outputs 2 and 3) and an info dict.
+    nargout=2..4 returns (image, F_number_tx_values, F_number_rx_values[, signal])
+    like the CPU path (das_sa.m:1; F per bin [N_f], no depth clamp, das_sa.m:354-358).
     """
     prefix = "das_sa"
     px, pz, rf, f_s, w, p, c, fb, t0 = _common_checks(prefix, positions_x, positions_z, data_RF, f_s,
@@ -264,6 +304,16 @@ def das_sa(positions_x, positions_z, data_RF, f_s, element_width, element_pitch,
     n_t, n_el = rf.shape[0], rf.shape[1]
     P, keep, info = sa_params(px, pz, n_t, n_el, f_s, w, p, c, fb, t0, wtx, wrx, Ftx, Frx, ntx, nrx, pf.index,
                               semantics, precision, output_scale, verbosity)
+    if nargout >= 2:
+        outs = (info["F_number_tx_values"].copy(), info["F_number_rx_values"].copy())
+        if nargout >= 4:
+            from .plan import SaPlan
+
+            plan = SaPlan(px, pz, n_t, n_el, f_s, w, p, c, fb, t0, window_tx=wtx, window_rx=wrx, F_number_tx=Ftx,
+                          F_number_rx=Frx, normalization_tx=ntx, normalization_rx=nrx, index_gpu=pf.index,
+                          semantics=semantics, precision=precision, output_scale=output_scale)
+            image, sig = _execute_with_signal(plan, rf, px.size, pz.size, pf.index)
+            return (image,) + outs + (sig,)
     rfF = np.asfortranarray(rf, dtype=np.float64 if rf.dtype == np.float64 else np.float32)
     dtype = _lib.DTYPE["float64" if rfF.dtype == np.float64 else "float32"]
     img = np.empty(2 * px.size * pz.size, dtype=np.float32)
@@ -275,6 +325,8 @@ def das_sa(positions_x, positions_z, data_RF, f_s, element_width, element_pitch,
                            wr.ctypes.data_as(_fp) if wr is not None else None), "gpu_bf_saft_rf")
     del keep
     image = _image_out(img, px.size, pz.size)
+    if nargout >= 2:
+        return (image,) + outs
     if return_weights:
         return image, wt.reshape(px.size, pz.size).T, wr.reshape(px.size, pz.size).T, info
     return image

@@ -70,6 +70,16 @@ class _Plan:
                    self.prefix)
         return {f: getattr(r, f) for f, _ in _lib.TuneResult._fields_}
 
+    def focused_signal(self, stream=None, device=None):
+        """Focused RF signal of the last voxel (das_pw.m:477-481 / das_sa.m:501-505) from the
+        spectra of the last execute: torch complex64 [N_dft] (CPU path scale)."""
+        import torch
+
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        out = torch.empty(self.info()["N_dft"], dtype=torch.complex64, device=dev)
+        _lib.check(_lib.load().fn_plan_focused_signal(self._h, _ptr(out), _stream(stream, dev)), self.prefix)
+        return out
+
     def set_tiling(self, m_tiles: int, bin_split: int):
         _lib.check(_lib.load().fn_plan_set_tiling(self._h, int(m_tiles), int(bin_split)), self.prefix)
 

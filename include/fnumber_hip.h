@@ -258,6 +258,15 @@ int32_t fn_batch_execute(fn_batch *batch, const void *const *d_rf, const int32_t
 int32_t fn_batch_info(const fn_batch *batch, int64_t *n_nodes, int32_t *n_captures);
 int32_t fn_batch_destroy(fn_batch *batch);
 
+/* Focused RF signal of the last voxel the CPU loop processes (third output of
+ * das_pw, das_pw.m:477-481; fourth of das_sa, das_sa.m:501-505; on the GPU the
+ * reference returns neither): the focused spectrum of that voxel -- normalized
+ * apodization x phase x analytic spectra, summed over the elements (SA: and the
+ * transmitters; PW plans with several angles: and the angles) -- placed at its
+ * bins and inverse-DFT'd to d_signal, complex float [N_dft], in the CPU path's
+ * scale.  Uses the spectra of the plan's last execute (same stream). */
+int32_t fn_plan_focused_signal(fn_plan *plan, float *d_signal, void *stream);
+
 /* ---- differentiation form (plane-wave plans) ---------------------------- */
 /* No counterpart in the reference (its MEX only beamforms); these extend the
  * plan of fn_plan_pw_create with the time-of-flight parameters

@@ -77,7 +77,16 @@ def _normalize(samples, normalization):
     # +normalizations/on.m:60-61 (sum over the element dimension 2)
     if normalization == "off":
         return samples
-    return samples / samples.sum(axis=1, keepdims=True)
+    samples_norm = samples / samples.sum(axis=1, keepdims=True)
+    if isinstance(normalization, tuple) and normalization[0] == "windowed":
+        # +normalizations/windowed.m:81-86 (rows: valid frequencies, columns: elements)
+        _, name, param = normalization
+        lengths_over_two = np.sum(samples_norm > 0, axis=0) / 2
+        with np.errstate(divide="ignore", invalid="ignore"):
+            temp = window_samples(name, param, (np.arange(samples.shape[0])[:, None] - lengths_over_two)
+                                  / lengths_over_two)
+            samples_norm = samples_norm * temp
+    return samples_norm
 
 
 def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width, element_pitch,

@@ -462,6 +462,23 @@ static double f_at(double F_n, double f_n, double z, double c_0, int clamp)
     return fmin(F_n, F_ub);
 }
 
+/* focused RF signal (das_pw.m:477-481): s[t] = ifft of the focused spectrum
+   placed at its bins, s[t] = (1/N) sum_i F[i] e^{+j 2 pi (n_lb + i) t / N} */
+static void signal_idft(const double *fr, const double *fi, int nf, int nlb, int N, double *out_re, double *out_im)
+{
+    for (int t = 0; t < N; ++t) {
+        double sr = 0.0, si = 0.0;
+        for (int i = 0; i < nf; ++i) {
+            const long m = ((long)(nlb + i) * t) % N;
+            const double ang = 2 * M_PI * (double)m / N, c = cos(ang), s = sin(ang);
+            sr += fr[i] * c - fi[i] * s;
+            si += fr[i] * s + fi[i] * c;
+        }
+        out_re[t] = sr / N;
+        out_im[t] = si / N;
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* das_pw                                                                    */
 /* ------------------------------------------------------------------------- */
@@ -497,6 +514,23 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
     const int clamp = (a->semantics == OR_SEM_CPU);
     const int nvox = a->n_sel > 0 ? a->n_sel : a->n_x * a->n_z;
     for (long v = 0; v < nvox; ++v) { a->image_re[v] = 0.0; a->image_im[v] = 0.0; }
+    /* the last voxel the CPU loop processes: last lateral position with a valid
+       frequency, its last such axial position (das_pw.m:393-474) */
+    long v_sig = -1;
+    double *sig_fr = NULL, *sig_fi = NULL;
+    if (a->signal_re && a->n_sel == 0) {
+        for (int ix = a->n_x - 1; ix >= 0 && v_sig < 0; --ix)
+            for (int iz = a->n_z - 1; iz >= 0 && v_sig < 0; --iz)
+                for (int i = 0; i < nf; ++i) {
+                    const double F = f_at(Fv[i], fax[i], a->z[iz], c0, clamp);
+                    if (aperture(a->x[ix], a->z[iz] / (2 * F), M, a->pitch, n_el, a->width).valid) {
+                        v_sig = (long)ix * a->n_z + iz;
+                        break;
+                    }
+                }
+        sig_fr = (double *)calloc(nf, sizeof(double));
+        sig_fi = (double *)calloc(nf, sizeof(double));
+    }
     const int want_dc = dx && dx->dIdc_re;
     const int want_vjp = dx && dx->g_re;
     const int want_adj = dx && dx->y_re;
@@ -553,7 +587,9 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
                 if (a->norm_rx != OR_NORM_OFF)
                     for (int k = 0; k < n_el; ++k) w[(long)j * n_el + k] = w[(long)j * n_el + k] / s;
             }
-            if (a->norm_rx == OR_NORM_WINDOWED) windowed_taper(w, nfv, n_el, a->norm_rx_win, a->norm_rx_win_param);
+            /* the MEX reads the taper window but never applies it (gpu_bf_das_pw_rf.cu:421-427) */
+            if (a->norm_rx == OR_NORM_WINDOWED && a->semantics == OR_SEM_CPU)
+                windowed_taper(w, nfv, n_el, a->norm_rx_win, a->norm_rx_win_param);
 
             double acc_r = 0.0, acc_i = 0.0, dc_r = 0.0, dc_i = 0.0;
             const double gr = want_vjp ? dx->g_re[v] : 0.0, gi = want_vjp ? dx->g_im[v] : 0.0;
@@ -605,6 +641,7 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
                     }
                     img_r += fr;
                     img_i += fi;
+                    if (v == v_sig) { sig_fr[i] += fr; sig_fi[i] += fi; }
                 }
                 acc_r += img_r;
                 acc_i += img_i;
@@ -640,6 +677,11 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
             }
         free(Zr_all);
         free(Zi_all);
+    }
+    if (sig_fr) {
+        signal_idft(sig_fr, sig_fi, nf, nlb, N, a->signal_re, a->signal_im);
+        free(sig_fr);
+        free(sig_fi);
     }
     free(fax); free(om); free(r); free(Fv); free(Xr); free(Xi);
     return 0;
@@ -696,6 +738,23 @@ int or_das_sa(or_das_args_t *a)
     or_spectra(a->rf, a->rf_f32, a->n_t, (int)nch, N, nlb, nub, a->spectra_mode, a->n_threads, Xr, Xi);
 
     const int nvox = a->n_sel > 0 ? a->n_sel : a->n_x * a->n_z;
+    /* last voxel the CPU loop processes (das_sa.m:375-498): both apertures valid */
+    long v_sig = -1;
+    double *sig_fr = NULL, *sig_fi = NULL;
+    if (a->signal_re && a->n_sel == 0) {
+        for (int ix = a->n_x - 1; ix >= 0 && v_sig < 0; --ix)
+            for (int iz = a->n_z - 1; iz >= 0 && v_sig < 0; --iz)
+                for (int i = 0; i < nf; ++i) {
+                    const double zz = a->z[iz], xx = a->x[ix];
+                    if (aperture(xx, zz / (2 * Ftx[i]), M, a->pitch, n_el, a->width).valid &&
+                        aperture(xx, zz / (2 * Frx[i]), M, a->pitch, n_el, a->width).valid) {
+                        v_sig = (long)ix * a->n_z + iz;
+                        break;
+                    }
+                }
+        sig_fr = (double *)calloc(nf, sizeof(double));
+        sig_fi = (double *)calloc(nf, sizeof(double));
+    }
 #ifdef _OPENMP
     if (a->n_threads > 0) omp_set_num_threads(a->n_threads);
 #endif
@@ -739,8 +798,13 @@ int or_das_sa(or_das_args_t *a)
                 vidx[nfv++] = i;
             }
             if (nfv == 0) { a->image_re[v] = 0.0; a->image_im[v] = 0.0; continue; }
-            if (a->norm_tx == OR_NORM_WINDOWED) windowed_taper(wtx_all, nfv, n_el, a->norm_tx_win, a->norm_tx_win_param);
-            if (a->norm_rx == OR_NORM_WINDOWED) windowed_taper(wrx_all, nfv, n_el, a->norm_rx_win, a->norm_rx_win_param);
+            /* MEX semantics: the taper window is read but not applied (gpu_bf_saft_rf.cu:485-490) */
+            if (a->semantics == OR_SEM_CPU) {
+                if (a->norm_tx == OR_NORM_WINDOWED)
+                    windowed_taper(wtx_all, nfv, n_el, a->norm_tx_win, a->norm_tx_win_param);
+                if (a->norm_rx == OR_NORM_WINDOWED)
+                    windowed_taper(wrx_all, nfv, n_el, a->norm_rx_win, a->norm_rx_win_param);
+            }
             /* das_sa.m:431 */
             const double zc = z / c0;
             for (int k = 0; k < n_el; ++k) {
@@ -777,12 +841,18 @@ int or_das_sa(or_das_args_t *a)
                 }
                 acc_r += fr;
                 acc_i += fi;
+                if (v == v_sig) { sig_fr[i] = fr; sig_fi[i] = fi; }
             }
             a->image_re[v] = acc_r;
             a->image_im[v] = acc_i;
         }
         free(tsc); free(wtx); free(wrx); free(ftr); free(fti); free(er); free(ei);
         free(wtx_all); free(wrx_all); free(vidx);
+    }
+    if (sig_fr) {
+        signal_idft(sig_fr, sig_fi, nf, nlb, N, a->signal_re, a->signal_im);
+        free(sig_fr);
+        free(sig_fi);
     }
     free(fax); free(om); free(r); free(Ftx); free(Frx); free(Xr); free(Xi);
     return 0;

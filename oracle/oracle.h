@@ -87,6 +87,10 @@ typedef struct {
     double *image_re, *image_im;
     /* grid info out */
     int N_dft, n_lb, n_ub;
+    /* optional (NULL: not computed; full grid only): focused RF signal of the last
+       voxel the CPU loop processes, [N_dft] complex (das_pw.m:477-481, das_sa.m:501-505),
+       angles summed for PW */
+    double *signal_re, *signal_im;
 } or_das_args_t;
 
 /* das_pw.m:302-474, compounded over the n_tx angles (examples.m:70-78). */

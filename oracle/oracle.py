@@ -58,6 +58,7 @@ class _Args(C.Structure):
         ("n_sel", C.c_int), ("sel_ix", C.POINTER(C.c_int)), ("sel_iz", C.POINTER(C.c_int)),
         ("image_re", C.POINTER(C.c_double)), ("image_im", C.POINTER(C.c_double)),
         ("N_dft", C.c_int), ("n_lb", C.c_int), ("n_ub", C.c_int),
+        ("signal_re", C.POINTER(C.c_double)), ("signal_im", C.POINTER(C.c_double)),
     ]
 
 
@@ -179,10 +180,11 @@ def _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics, spect
 def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
            window=("tukey", 0.2), F=None, normalization="on", semantics="cpu",
            spectra_mode=1, n_threads=0, sel=None, norm_window=("boxcar", 0.0),
-           c_tof=None, delay_offsets=None):
+           c_tof=None, delay_offsets=None, want_signal=False):
     """rf: [n_tx, n_el, n_t] C-order (MATLAB [n_t x n_el x n_tx]).  Returns (image, info).
 
-    image is [n_z, n_x] complex (or [n_sel] with ``sel=(ix, iz)``), CPU scale."""
+    image is [n_z, n_x] complex (or [n_sel] with ``sel=(ix, iz)``), CPU scale.
+    want_signal: info['signal'] = focused RF signal of the last voxel, [N_dft] complex."""
     rf = np.asarray(rf)
     if rf.ndim == 2:
         rf = rf[None]
@@ -197,6 +199,7 @@ def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
     a.f_rx = F if F is not None else fnumber("angle_lb", 60, 1.5, 10)
     a.norm_rx = NORM[normalization]
     a.norm_rx_win, a.norm_rx_win_param = WIN[norm_window[0]], float(norm_window[1])
+    sig = _signal_buffers(a, keep, n_t, want_signal and sel is None)
     if c_tof is None:
         rc = lib().or_das_pw(C.byref(a))
     else:
@@ -210,7 +213,24 @@ def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
     img = re + 1j * im
     if sel is None:
         img = img.reshape(np.asarray(x).size, np.asarray(z).size).T
-    return img, dict(N_dft=a.N_dft, n_lb=a.n_lb, n_ub=a.n_ub)
+    info = dict(N_dft=a.N_dft, n_lb=a.n_lb, n_ub=a.n_ub)
+    if sig is not None:
+        info["signal"] = (sig[0] + 1j * sig[1])[:a.N_dft]
+    return img, info
+
+
+def _signal_buffers(a, keep, n_t, want):
+    """signal outputs of the DFT order of the call's grid (das_pw.m:343-353)"""
+    a.signal_re = a.signal_im = None
+    if not want:
+        return None
+    z = np.ctypeslib.as_array(a.z, shape=(a.n_z,))
+    z_pad = z[1] if (a.semantics == SEM["mex"] and a.n_z > 1) else z[0]
+    N, _, _ = grid(int(n_t), a.n_el, a.pitch, float(z_pad), a.f_s, a.c_0, a.f_lb, a.f_ub)
+    sr, si = np.zeros(N), np.zeros(N)
+    a.signal_re, a.signal_im = _dp(sr), _dp(si)
+    keep += [sr, si]
+    return sr, si
 
 
 class _Diff(C.Structure):
@@ -290,7 +310,7 @@ def das_pw_diff(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0
 def das_sa(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0=0.0,
            window_tx=("tukey", 0.2), window_rx=("tukey", 0.2), F_tx=None, F_rx=None,
            normalization_tx="on", normalization_rx="on", semantics="cpu", spectra_mode=1,
-           n_threads=0, sel=None, norm_window=("boxcar", 0.0)):
+           n_threads=0, sel=None, norm_window=("boxcar", 0.0), want_signal=False):
     """rf: [n_tx, n_rx, n_t] C-order (MATLAB [n_t x n_rx x n_tx])."""
     rf = np.asarray(rf)
     n_tx, n_rx, n_t = rf.shape
@@ -306,10 +326,14 @@ def das_sa(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0=0.0,
     a.norm_rx = NORM[normalization_rx]
     a.norm_tx_win = a.norm_rx_win = WIN[norm_window[0]]
     a.norm_tx_win_param = a.norm_rx_win_param = float(norm_window[1])
+    sig = _signal_buffers(a, keep, n_t, want_signal and sel is None)
     rc = lib().or_das_sa(C.byref(a))
     if rc:
         raise ValueError(f"or_das_sa failed ({rc})")
     img = re + 1j * im
     if sel is None:
         img = img.reshape(np.asarray(x).size, np.asarray(z).size).T
-    return img, dict(N_dft=a.N_dft, n_lb=a.n_lb, n_ub=a.n_ub)
+    info = dict(N_dft=a.N_dft, n_lb=a.n_lb, n_ub=a.n_ub)
+    if sig is not None:
+        info["signal"] = (sig[0] + 1j * sig[1])[:a.N_dft]
+    return img, info

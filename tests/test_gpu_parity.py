@@ -131,3 +131,154 @@ def test_das_pw_full_size_voxel_subset(fa, oracle):
     print(f"  N_dft={info['N_dft']} band={info['n_lb']}..{info['n_ub']} rel_l2={e:.3e}", flush=True)
     assert info["N_dft"] == 2509 and info["n_lb"] == 271 and info["n_ub"] == 812
     assert e <= TOL_L2
+
+
+# ---- windowed normalization (windowed.m:81-86) -------------------------------
+# Tapers whose value at t = NaN / inf is 0 (Tukey, boxcar) agree with MATLAB on
+# element columns without positive rows (L = 0, where MATLAB forms 0/0); Hann and
+# triangular tapers are checked with the Hann receive window, whose columns all
+# have positive rows.
+NW_CASES = [("tukey", ("tukey", 0.2)), ("boxcar", ("tukey", 0.2)), ("triangular", ("tukey", 0.2)),
+            ("hann", ("tukey", 0.2)), ("tukey", ("boxcar", 0.0)), ("hann", ("hann", 0.0)),
+            ("hann", ("triangular", 0.0))]
+
+
+def _taper(fa, spec):
+    name, p = spec
+    return {"tukey": lambda: fa.windows.tukey(p), "boxcar": lambda: fa.windows.boxcar(),
+            "hann": lambda: fa.windows.hann(), "triangular": lambda: fa.windows.triangular()}[name]()
+
+
+@pytest.mark.parametrize("semantics", ["cpu", "mex"])
+@pytest.mark.parametrize("win,taper", NW_CASES, ids=[f"{w}-{t[0]}{t[1]}" for w, t in NW_CASES])
+def test_das_pw_windowed_normalization(fa, oracle, small_pw, win, taper, semantics):
+    w, rf = small_pw
+    Fo = fa.f_numbers.grating.angle_lb(60, 1.5)
+    print(f"case das_pw windowed win={win} taper={taper} sem={semantics}", flush=True)
+    img = fa.das_pw(w.positions_x, w.positions_z, np.transpose(rf, (2, 1, 0)), w.f_s, w.angles, w.width,
+                    w.pitch, w.c_0, w.f_bounds, 0, _win(fa, win), Fo,
+                    fa.normalizations.windowed(_taper(fa, taper)), fa.platforms.gpu(0), semantics=semantics,
+                    output_scale="cpu")
+    ref, info = oracle.das_pw(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0,
+                              w.f_bounds, 0, (win, 0.5 if win == "tukey" else 0.0), _orc_F(oracle, Fo), "windowed",
+                              semantics=semantics, norm_window=taper)
+    assert np.all(np.isfinite(ref))
+    e, d = rel_l2(img, ref), peak_db(img, ref)
+    print(f"  rel_l2={e:.3e} peak={d:.1f} dB", flush=True)
+    assert e <= TOL_L2 and d <= TOL_DB
+
+
+@pytest.mark.parametrize("nw_tx,nw_rx", [(False, True), (True, False), (True, True)])
+def test_das_sa_windowed_normalization(fa, oracle, nw_tx, nw_rx):
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.saft_full(), 12, 10, n_samples=1024, n_speckle=600, n_elements=32)
+    w.pitch = 4 * 0.3048e-3 / 1.0
+    w.width = 4 * 0.27e-3
+    rf = w.rf_host()
+    Fo = fa.f_numbers.grating.angle_lb(60, 1.5)
+    nwin = fa.windows.tukey(0.2)
+    ntx = fa.normalizations.windowed(nwin) if nw_tx else fa.normalizations.on()
+    nrx = fa.normalizations.windowed(nwin) if nw_rx else fa.normalizations.on()
+    print(f"case das_sa windowed tx={nw_tx} rx={nw_rx}", flush=True)
+    img = fa.das_sa(w.positions_x, w.positions_z, np.transpose(rf, (2, 1, 0)), w.f_s, w.width, w.pitch, w.c_0,
+                    w.f_bounds, 0, fa.windows.tukey(0.5), fa.windows.tukey(0.5), Fo, Fo, ntx, nrx,
+                    output_scale="cpu")
+    ref, info = oracle.das_sa(w.positions_x, w.positions_z, rf, w.f_s, w.width, w.pitch, w.c_0, w.f_bounds, 0,
+                              ("tukey", 0.5), ("tukey", 0.5), oracle.fnumber("angle_lb", 60, 1.5, 10),
+                              oracle.fnumber("angle_lb", 60, 1.5, 10), "windowed" if nw_tx else "on",
+                              "windowed" if nw_rx else "on", norm_window=("tukey", 0.2))
+    e, d = rel_l2(img, ref), peak_db(img, ref)
+    print(f"  rel_l2={e:.3e} peak={d:.1f} dB", flush=True)
+    assert e <= TOL_L2 and d <= TOL_DB
+
+
+def test_windowed_normalization_limits(fa, small_pw):
+    """The taper path is FP32 and not differentiated: other precisions and the
+    differentiation entry points report Unsupported."""
+    w, rf = small_pw
+    nrm = fa.normalizations.windowed(fa.windows.tukey(0.2))
+    with pytest.raises(fa.FNumberError, match="Unsupported"):
+        fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch, w.c_0,
+                  w.f_bounds, 0.0, normalization=nrm, precision="fp16")
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     w.c_0, w.f_bounds, 0.0, normalization=nrm)
+    import torch
+
+    x = torch.from_numpy(rf).cuda()
+    img = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device="cuda")
+    with pytest.raises(fa.FNumberError, match="Unsupported"):
+        plan.execute_tangent(x, img, torch.empty_like(img))
+    plan.close()
+
+
+# ---- focused RF signal of the last voxel (das_pw.m:477-481, das_sa.m:501-505) --
+@pytest.mark.parametrize("win,norm,sem", [("tukey", "on", "cpu"), ("boxcar", "off", "mex"), ("hann", "on", "cpu"),
+                                          ("tukey", "windowed", "cpu")])
+def test_focused_signal_matches_oracle(fa, oracle, small_pw, win, norm, sem):
+    import torch
+
+    w, rf = small_pw
+    Fo = fa.f_numbers.grating.angle_lb(60, 1.5)
+    nrm = {"on": fa.normalizations.on(), "off": fa.normalizations.off(),
+           "windowed": fa.normalizations.windowed(fa.windows.tukey(0.2))}[norm]
+    print(f"case focused signal win={win} norm={norm} sem={sem}", flush=True)
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     w.c_0, w.f_bounds, 0.0, window=_win(fa, win), F_number=Fo, normalization=nrm, semantics=sem,
+                     output_scale="cpu")
+    x = torch.from_numpy(rf).cuda()
+    img = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device="cuda")
+    plan.execute(x, img)
+    sig = plan.focused_signal().cpu().numpy()
+    plan.close()
+    _, info = oracle.das_pw(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0, w.f_bounds,
+                            0, (win, 0.5 if win == "tukey" else 0.0), _orc_F(oracle, Fo), norm, semantics=sem,
+                            norm_window=("tukey", 0.2), want_signal=True)
+    e = rel_l2(sig, info["signal"])
+    print(f"  rel_l2={e:.3e} (N_dft {info['N_dft']})", flush=True)
+    assert sig.shape == info["signal"].shape and e <= TOL_L2
+
+
+def test_focused_signal_sa_matches_oracle(fa, oracle):
+    import torch
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.saft_full(), 12, 10, n_samples=1024, n_speckle=600, n_elements=32)
+    w.pitch = 4 * 0.3048e-3 / 1.0
+    w.width = 4 * 0.27e-3
+    rf = w.rf_host()
+    Fo = fa.f_numbers.grating.angle_lb(60, 1.5)
+    print("case focused signal SA", flush=True)
+    plan = fa.SaPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.f_s, w.width, w.pitch, w.c_0,
+                     w.f_bounds, 0.0, window_tx=fa.windows.tukey(0.5), window_rx=fa.windows.tukey(0.5),
+                     F_number_tx=Fo, F_number_rx=Fo, output_scale="cpu")
+    x = torch.from_numpy(rf).cuda()
+    img = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device="cuda")
+    plan.execute(x, img)
+    sig = plan.focused_signal().cpu().numpy()
+    plan.close()
+    _, info = oracle.das_sa(w.positions_x, w.positions_z, rf, w.f_s, w.width, w.pitch, w.c_0, w.f_bounds, 0,
+                            ("tukey", 0.5), ("tukey", 0.5), want_signal=True)
+    e = rel_l2(sig, info["signal"])
+    print(f"  rel_l2={e:.3e}", flush=True)
+    assert e <= TOL_L2
+
+
+def test_das_pw_nargout_matches_cpu_outputs(fa, oracle, small_pw):
+    """[image, F_number_values, signal] = das_pw(...) (das_pw.m:1): F per bin and depth
+    with the depth clamp, the signal of the last voxel, the image of the plain call."""
+    w, rf = small_pw
+    args = (w.positions_x, w.positions_z, np.transpose(rf, (2, 1, 0)), w.f_s, w.angles, w.width, w.pitch, w.c_0,
+            w.f_bounds, 0, fa.windows.tukey(0.5), fa.f_numbers.grating.angle_lb(60, 1.5))
+    img3, Fv, sig = fa.das_pw(*args, nargout=3, output_scale="cpu")
+    img1 = fa.das_pw(*args, output_scale="cpu")
+    img2, Fv2 = fa.das_pw(*args, nargout=2, output_scale="cpu")
+    assert rel_l2(img3, img1) < 1e-6 and np.array_equal(Fv, Fv2)
+    N = sig.size
+    assert Fv.shape[1] == w.positions_z.size
+    # the signal's first sample is the last voxel's image value / N_dft (sum of its focused spectrum)
+    assert abs(N * sig[0] - img1[-1, -1]) <= 1e-5 * np.max(np.abs(img1))
+    ref, info = oracle.das_pw(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0,
+                              w.f_bounds, 0, ("tukey", 0.5), oracle.fnumber("angle_lb", 60, 1.5, 10), "on",
+                              want_signal=True)
+    assert rel_l2(sig, info["signal"]) <= TOL_L2

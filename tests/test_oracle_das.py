@@ -137,3 +137,73 @@ def test_saft_single_transmitter(oracle):
             tau = np.sqrt((xk - xv) ** 2 + zv ** 2) / C0
             ref[iz, ix] = np.sum(np.exp(1j * np.outer(om, tau[tx] + tau)) * 2 * X.T)
     assert np.linalg.norm(img - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def _sig_case():
+    rng = np.random.default_rng(11)
+    x = np.linspace(-2e-3, 2e-3, 7)
+    z = np.linspace(3e-3, 8e-3, 5)
+    rf = rng.normal(size=(3, 16, 256)).astype(np.float32)
+    return x, z, np.deg2rad([-4.0, 0.0, 4.0]), rf
+
+
+@pytest.mark.parametrize("win,norm,sem", [(("tukey", 0.5), "on", "cpu"), (("boxcar", 0.0), "off", "mex"),
+                                          (("hann", 0.0), "windowed", "cpu")])
+def test_oracle_focused_signal_pins(oracle, win, norm, sem):
+    """das_pw.m:477-481: the signal is the ifft of the last voxel's focused spectrum.
+    Pins: (i) its image value is the focused spectrum summed over the bins, i.e.
+    image(last) = N_dft * s[0]; (ii) the signal is band-limited: N fft(s) vanishes
+    outside [n_lb, n_ub]."""
+    x, z, ang, rf = _sig_case()
+    img, info = oracle.das_pw(x, z, rf, 20.832e6, ang, 0.27e-3, 0.3048e-3, 1540.0, (2.25e6, 6.75e6), 0.0, win,
+                              oracle.fnumber("angle_lb", 60, 1.5, 10), norm, sem, norm_window=("tukey", 0.2),
+                              want_signal=True)
+    s, N = info["signal"], info["N_dft"]
+    assert s.shape == (N,)
+    last = img[-1, -1]
+    assert abs(N * s[0] - last) <= 1e-12 * abs(last)
+    S = np.fft.fft(s)
+    band = np.zeros(N, bool)
+    band[info["n_lb"]:info["n_ub"] + 1] = True
+    assert np.max(np.abs(S[~band])) <= 1e-12 * np.max(np.abs(S[band]))
+
+
+def test_oracle_focused_signal_pins_sa(oracle):
+    rng = np.random.default_rng(12)
+    x = np.linspace(-1e-3, 1e-3, 5)
+    z = np.linspace(3e-3, 7e-3, 4)
+    rf = rng.normal(size=(8, 8, 256)).astype(np.float32)
+    img, info = oracle.das_sa(x, z, rf, 20.832e6, 0.27e-3, 0.3048e-3, 1540.0, (2.25e6, 6.75e6), 0.0,
+                              ("tukey", 0.5), ("tukey", 0.5), want_signal=True)
+    s, N = info["signal"], info["N_dft"]
+    last = img[-1, -1]
+    assert abs(N * s[0] - last) <= 1e-12 * abs(last)
+    S = np.fft.fft(s)
+    band = np.zeros(N, bool)
+    band[info["n_lb"]:info["n_ub"] + 1] = True
+    assert np.max(np.abs(S[~band])) <= 1e-12 * np.max(np.abs(S[band]))
+
+
+@pytest.mark.parametrize("win,taper", [(("tukey", 0.5), ("tukey", 0.2)), (("boxcar", 0.0), ("tukey", 0.2)),
+                                       (("hann", 0.0), ("hann", 0.0)), (("triangular", 0.0), ("boxcar", 0.0))])
+def test_oracle_windowed_equals_numpy_transliteration(oracle, win, taper):
+    """normalizations.windowed (windowed.m:81-86) in the C oracle against the numpy
+    transliteration: 'on', then each element column tapered over the valid rows by
+    window((i - L/2) / (L/2)), L = number of positive entries of the column."""
+    rng = np.random.default_rng(13)
+    n_el, n_t = 16, 256
+    x = np.linspace(-3e-3, 3e-3, 7)
+    z = np.linspace(5e-3, 12e-3, 9)
+    rf = rng.standard_normal((n_t, n_el)).astype(np.float32)
+    F = oracle.fnumber("angle_lb", 60, 1.5, 10)
+    for ang, t0 in [(0.0, 0), (0.12, 3)]:
+        ref = DN.das_pw(x, z, rf.astype(np.float64), FS, ang, W, P, C0, (2.25e6, 6.75e6), t0, win, _fn_np("angle_lb"),
+                        ("windowed",) + taper)
+        img, _ = oracle.das_pw(x, z, rf.T[None], FS, [ang], W, P, C0, (2.25e6, 6.75e6), t0, win, F, "windowed",
+                               norm_window=taper)
+        fin = np.isfinite(ref)
+        assert np.array_equal(fin, np.isfinite(img))
+        assert np.linalg.norm(img[fin] - ref[fin]) <= 1e-13 * np.linalg.norm(ref[fin])
+        # the taper changes the image (it is applied)
+        on, _ = oracle.das_pw(x, z, rf.T[None], FS, [ang], W, P, C0, (2.25e6, 6.75e6), t0, win, F, "on")
+        assert np.linalg.norm(img[fin] - on[fin]) > 1e-3 * np.linalg.norm(on[fin])

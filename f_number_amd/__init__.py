@@ -8,7 +8,7 @@ from . import f_numbers, normalizations, platforms, windows  # noqa: F401
 from ._lib import FNumberError, LIB_PATH, load  # noqa: F401
 from .das import das_pw, das_sa  # noqa: F401
 from .plan import Batch, PwPlan, SaPlan  # noqa: F401
-from . import autofocus, synth, tuning  # noqa: F401,E402
+from . import autofocus, examples, synth, tuning  # noqa: F401,E402
 
 __all__ = ["das_pw", "das_sa", "PwPlan", "SaPlan", "windows", "f_numbers", "normalizations", "platforms",
            "FNumberError", "load"]

@@ -48,6 +48,7 @@ class PwParams(C.Structure):
         ("normalization", C.c_int32), ("normalization_window", Window),
         ("semantics", C.c_int32), ("precision", C.c_int32), ("output_scale", C.c_int32),
         ("index_gpu", C.c_int32), ("verbosity", C.c_int32), ("n_frames", C.c_int32),
+        ("virtual_sources", _dp),
     ]
 
 
@@ -90,6 +91,7 @@ class SynthParams(C.Structure):
         ("layer_thickness", C.c_double), ("layer_c", C.c_double), ("delay_offsets", _dp),
         ("tube_x", C.c_double), ("tube_z", C.c_double), ("tube_radius", C.c_double), ("tube_angle", C.c_double),
         ("tube_v_max", C.c_double), ("frame_time", C.c_double),
+        ("virtual_sources", _dp),
     ]
 
 

@@ -240,10 +240,7 @@ __global__ __launch_bounds__(256) void adjoint_tables_kernel(BeamArgs A, uint32_
     } else {
         const int a = col - 128;
         double tau = 0.0;
-        if (a < A.n_tx) {
-            const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
-            tau = (ex * (x - xr) + ez * z) / A.ctof + A.t0 / A.f_s;
-        }
+        if (a < A.n_tx) tau = pw_tx_time(A, a, x, z) + A.t0 / A.f_s;
         qtx[v * 16 * n_at + a] = frac32(A.f_s * tau / A.N);
     }
 }

@@ -584,11 +584,21 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
              p->f_s, p->element_width, p->element_pitch, p->c_0, p->index_t0,
              p->semantics, p->precision, p->output_scale, p->index_gpu, p->verbosity};
     if ((s = create_common(P, c, g, p->f_number_values, nullptr))) { destroy(P); return s; }
-    // steering geometry in FP64: das_pw.m:323-327 (x_ref = sign(e_x) x_0)
+    // steering geometry in FP64: das_pw.m:323-327 (x_ref = sign(e_x) x_0), or the
+    // virtual sources (x_vs, z_vs, |r_vs|)
     std::vector<double> txp(3 * (size_t)n_rows);
     const double M = (p->n_elements - 1) / 2.0;
+    P->args.tx_kind = p->virtual_sources ? fnk::TX_VIRTUAL : fnk::TX_PLANE;
     for (int row = 0; row < n_rows; ++row) {
-        const double th = p->steering_angles[row % p->n_angles];
+        const int a = row % p->n_angles;
+        if (p->virtual_sources) {
+            const double xv = p->virtual_sources[2 * a], zv = p->virtual_sources[2 * a + 1];
+            txp[3 * row] = xv;
+            txp[3 * row + 1] = zv;
+            txp[3 * row + 2] = std::sqrt(xv * xv + zv * zv);
+            continue;
+        }
+        const double th = p->steering_angles[a];
         const double ex = -std::sin(th), ez = std::cos(th);
         const double sg = (ex > 0) - (ex < 0);
         txp[3 * row] = ex;

@@ -293,13 +293,22 @@ int validate_pw(const fn_pw_params *p, Grid *g)
                          p->element_width, p->element_pitch, p->c_0, p->f_lb, p->f_ub, p->index_t0, p->semantics,
                          p->precision, p->output_scale, p->index_gpu, p->verbosity);
     if (s) return s;
-    if (p->n_angles < 1 || !p->steering_angles) return fail(FN_ERR_STEERING_ANGLE, "Steering angle missing");
+    if (p->n_angles < 1 || (!p->steering_angles && !p->virtual_sources))
+        return fail(FN_ERR_STEERING_ANGLE, "Steering angle missing");
     if (p->n_frames < 0 || (long)std::max(1, p->n_frames) * p->n_angles > (1L << 24))
         return fail(FN_ERR_ARGUMENT, "n_frames must be in 0..2^24 / n_angles (got %d)", p->n_frames);
-    for (int a = 0; a < p->n_angles; ++a) {
-        const double th = p->steering_angles[a];
-        if (!(th > -M_PI / 2 && th < M_PI / 2))
-            return fail(FN_ERR_STEERING_ANGLE, "Steering angle must be larger than negative pi over two and smaller than pi over two!");
+    if (p->virtual_sources) {
+        for (int a = 0; a < p->n_angles; ++a) {
+            const double xv = p->virtual_sources[2 * a], zv = p->virtual_sources[2 * a + 1];
+            if (!(std::isfinite(xv) && std::isfinite(zv) && zv != 0.0))
+                return fail(FN_ERR_STEERING_ANGLE, "virtual source %d must be finite with z_vs != 0", a);
+        }
+    } else {
+        for (int a = 0; a < p->n_angles; ++a) {
+            const double th = p->steering_angles[a];
+            if (!(th > -M_PI / 2 && th < M_PI / 2))
+                return fail(FN_ERR_STEERING_ANGLE, "Steering angle must be larger than negative pi over two and smaller than pi over two!");
+        }
     }
     if ((s = check_window(p->window, FN_ERR_WINDOW))) return s;
     if (p->semantics == FN_SEMANTICS_CPU && p->window.id == FN_WINDOW_COSINE)

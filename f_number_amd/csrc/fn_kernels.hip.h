@@ -85,6 +85,7 @@ struct BeamArgs {
     int atomic_out;
     double M, pitch, width, c0, f_s, t0;
     int rows_per_frame;     // ensemble: transmit rows per frame (angles); images [frame][n_vox]
+    int tx_kind;            // TX_PLANE: txp = (e_x, e_z, x_ref); TX_VIRTUAL: txp = (x_vs, z_vs, |r_vs|)
     double ctof;            // speed of sound of the time of flight (c0 unless set)
     const double *delay;    // per-element delay offsets (s), [n_el]
     float *dimage;          // tangent: dI / dc_tof, interleaved complex [n_vox]
@@ -251,6 +252,23 @@ __device__ __forceinline__ float window_any(float t, const WinCoef &w)
 __device__ __forceinline__ float taper_at(int row, float h, const WinCoef &w)
 {
     return h > 0.f ? window_any(((float)row - h) / h, w) : 0.f;
+}
+
+// transmit models of the plane-wave mode
+enum { TX_PLANE = 0, TX_VIRTUAL = 1 };
+
+// geometric transmit time of row a at voxel (x, z), without the i_t0 / f_s shift:
+// plane wave (das_pw.m:336) (e_x (x - x_ref) + e_z z) / c; virtual source at
+// r_vs = (x_vs, z_vs): diverging (z_vs < 0) (|r - r_vs| - |r_vs|) / c, zero where
+// the wavefront crosses the array centre; focused (z_vs > 0)
+// (|r_vs| +- |r - r_vs|) / c, through the focus at |r_vs| / c (+ below it).
+__host__ __device__ __forceinline__ double pw_tx_time(const BeamArgs &A, int a, double x, double z)
+{
+    const double p0 = A.txp[3 * a], p1 = A.txp[3 * a + 1], p2 = A.txp[3 * a + 2];
+    if (A.tx_kind == TX_PLANE) return (p0 * (x - p2) + p1 * z) / A.ctof;
+    const double dx = x - p0, dz = z - p1;
+    const double r = sqrt(dx * dx + dz * dz);
+    return (p1 < 0.0 ? r - p2 : p2 + (z >= p1 ? r : -r)) / A.ctof;
 }
 
 // -------------------------------------------------------------------------
@@ -462,9 +480,8 @@ void beamform_kernel(BeamArgs A)
             double tau = 0.0, tg = 0.0;
             if (a < A.n_tx) {
                 if (MODE == MODE_PW) {
-                    // das_pw.m:336: (e_x (x - x_ref) + e_z z) / c + i_t0 / f_s
-                    const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
-                    tg = (ex * (x - xr) + ez * z) / A.ctof;
+                    // das_pw.m:336 (or a virtual source) + i_t0 / f_s
+                    tg = pw_tx_time(A, a, x, z);
                     tau = tg + A.t0 / A.f_s;
                 } else {
                     // transmitting element a: its geometric delay and its delay offset

@@ -94,8 +94,7 @@ __global__ __launch_bounds__(128) void focused_spectrum_kernel(BeamArgs A, SigAr
     float ar = 0.f, ai = 0.f;
     if (S.mode == MODE_PW) {
         for (int a = 0; a < A.n_tx; ++a) {
-            const double ex = A.txp[3 * a], ez = A.txp[3 * a + 1], xr = A.txp[3 * a + 2];
-            const uint32_t qtx = frac32(A.f_s * ((ex * (x - xr) + ez * z) / A.ctof + A.t0 / A.f_s) / A.N);
+            const uint32_t qtx = frac32(A.f_s * (pw_tx_time(A, a, x, z) + A.t0 / A.f_s) / A.N);
             if (!elem) continue;
             const float rv = rev_of(n * qrx + n * qtx);
             const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);

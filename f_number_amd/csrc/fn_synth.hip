@@ -8,6 +8,8 @@
 // fractional_bandwidth, placed at the exact transmit + receive delay:
 //   plane wave a:  tau = (e_x (x_s - x_ref) + e_z z_s) / c + |r_k - r_s| / c   (das_pw.m:336,427)
 //   element i -> k: tau = |r_i - r_s| / c + |r_k - r_s| / c                     (das_sa.m:431)
+//   virtual source (x_v, z_v): diverging (z_v < 0) (|r_s - r_v| - |r_v|) / c,
+//                  focused (z_v > 0) (|r_v| + sign(z_s - z_v) |r_s - r_v|) / c  (+ receive)
 // Layered medium (straight rays): a path of length L from the surface to depth
 // z_s spends the share f = min(1, h / z_s) of its length in the layer, so its
 // time is L / c_0 + f L (1 / c_layer - 1 / c_0); the plane-wave transmit path to
@@ -69,6 +71,11 @@ __host__ __device__ inline double tx_delay(const Geo &G, const double *txp, int 
         double t = (ex * (xs - xr) + ez * zs) / G.c0;
         if (G.h > 0.0) t += fmin(zs, G.h) / ez * G.slow;
         return t;
+    }
+    if (G.mode == 2) {  // virtual source (x_v, z_v, |r_v|), homogeneous medium
+        const double xv = txp[3 * a], zv = txp[3 * a + 1], r0 = txp[3 * a + 2];
+        const double r = sqrt((xs - xv) * (xs - xv) + (zs - zv) * (zs - zv));
+        return (zv < 0.0 ? r - r0 : r0 + (zs >= zv ? r : -r)) / G.c0;
     }
     const double dx = ((double)a - G.M) * G.pitch - xs;
     return path_time(G, sqrt(dx * dx + zs * zs), zs);
@@ -190,6 +197,19 @@ int build_geo(const fn_synth_params *p, Geo *G, std::vector<double> *txp)
             (*txp)[3 * a] = ex;
             (*txp)[3 * a + 1] = ez;
             (*txp)[3 * a + 2] = sg * ((0 - G->M) * p->element_pitch);
+        }
+    } else if (p->mode == 2) {
+        if (p->n_angles < 1 || !p->virtual_sources) return fail(FN_ERR_STEERING_ANGLE, "virtual sources missing");
+        if (p->layer_thickness > 0.0) return fail(FN_ERR_ARGUMENT, "virtual sources: homogeneous media only");
+        G->n_tx = p->n_angles;
+        txp->resize(3 * (size_t)p->n_angles);
+        for (int a = 0; a < p->n_angles; ++a) {
+            const double xv = p->virtual_sources[2 * a], zv = p->virtual_sources[2 * a + 1];
+            if (!(std::isfinite(xv) && std::isfinite(zv) && zv != 0.0))
+                return fail(FN_ERR_STEERING_ANGLE, "virtual source %d must be finite with z_vs != 0", a);
+            (*txp)[3 * a] = xv;
+            (*txp)[3 * a + 1] = zv;
+            (*txp)[3 * a + 2] = std::sqrt(xv * xv + zv * zv);
         }
     } else if (p->mode == 1) {
         G->n_tx = p->n_elements;

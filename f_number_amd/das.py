@@ -122,8 +122,10 @@ def _image_out(buf, n_x, n_z):
 
 
 def pw_params(px, pz, n_t, n_el, angles, f_s, w, p, c, fb, t0, window, F, normalization, index_gpu=0,
-              semantics="cpu", precision="fp32", output_scale="mex", verbosity=0, n_frames=1):
-    """Build the fn_pw_params struct (and the arrays it points to) for libfnumber_hip."""
+              semantics="cpu", precision="fp32", output_scale="mex", verbosity=0, n_frames=1,
+              virtual_sources=None):
+    """Build the fn_pw_params struct (and the arrays it points to) for libfnumber_hip.
+    virtual_sources: (N_a, 2) array of (x_vs, z_vs) in m replacing the steering angles."""
     L = _lib.load()
     px = np.ascontiguousarray(px, dtype=np.float64)
     pz = np.ascontiguousarray(pz, dtype=np.float64)
@@ -146,6 +148,12 @@ def pw_params(px, pz, n_t, n_el, angles, f_s, w, p, c, fb, t0, window, F, normal
     P.index_gpu = int(index_gpu)
     P.verbosity = int(verbosity)
     P.n_frames = int(n_frames)
+    vs = None
+    if virtual_sources is not None:
+        vs = np.ascontiguousarray(np.asarray(virtual_sources, dtype=np.float64).reshape(-1, 2))
+        if vs.shape[0] != angles.size:
+            _err("gpu_bf_das_pw_rf:InvalidSteeringAngle", "one virtual source (x_vs, z_vs) per transmit is required")
+        P.virtual_sources = vs.ctypes.data_as(_dp)
     N, lb, ub = C.c_int32(), C.c_int32(), C.c_int32()
     _lib.check(L.fn_pw_frequency_grid(C.byref(P), C.byref(N), C.byref(lb), C.byref(ub)), "gpu_bf_das_pw_rf")
     r = np.empty(ub.value - lb.value + 1)
@@ -153,7 +161,7 @@ def pw_params(px, pz, n_t, n_el, angles, f_s, w, p, c, fb, t0, window, F, normal
                "gpu_bf_das_pw_rf")
     Fv = np.ascontiguousarray(F.compute_values(r), dtype=np.float64)
     P.f_number_values, P.n_f = Fv.ctypes.data_as(_dp), Fv.size
-    keep = (px, pz, angles, Fv)
+    keep = (px, pz, angles, Fv, vs)
     return P, keep, dict(N_dft=N.value, n_lb=lb.value, n_ub=ub.value, F_number_values=Fv)
 
 
@@ -194,7 +202,7 @@ def sa_params(px, pz, n_t, n_el, f_s, w, p, c, fb, t0, window_tx, window_rx, F_t
 def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width, element_pitch, c_0,
            f_bounds=None, index_t0=None, window=None, F_number=None, normalization=None, platform=None, *,
            semantics: str = "cpu", precision: str = "fp32", output_scale: str = "mex",
-           return_weights: bool = False, verbosity: int = 0, nargout: int = 1):
+           return_weights: bool = False, verbosity: int = 0, nargout: int = 1, virtual_sources=None):
     """Fourier-domain DAS of steered plane waves (das_pw.m:1).
 
     Returns the complex64 image [N_z, N_x] in the MEX scale 2/N_dft * sum
@@ -207,6 +215,9 @@ def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width
     (das_pw.m:1): the F-numbers per bin and depth [N_f, N_z] (clamped at
     sqrt(f z / 2.88 c) in CPU semantics, das_pw.m:374-377) and the focused RF
     signal of the last voxel [N_dft] (das_pw.m:477-481; angles summed).
+    virtual_sources: (N_a, 2) (x_vs, z_vs) in m per transmit instead of the plane
+    waves of steering_angle (which then only gives N_a): diverging (z_vs < 0) or
+    focused (z_vs > 0) waves (fnumber_hip.h fn_pw_params.virtual_sources).
     """
     prefix = "das_pw"
     px, pz, rf, f_s, w, p, c, fb, t0 = _common_checks(prefix, positions_x, positions_z, data_RF, f_s,
@@ -226,7 +237,7 @@ def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width
     pf = _platform(platform, prefix)
     n_t, n_el = rf.shape[0], rf.shape[1]
     P, keep, info = pw_params(px, pz, n_t, n_el, ang, f_s, w, p, c, fb, t0, win, F, nrm, pf.index,
-                              semantics, precision, output_scale, verbosity)
+                              semantics, precision, output_scale, verbosity, virtual_sources=virtual_sources)
     if nargout >= 2:
         f = f_s * np.arange(info["n_lb"], info["n_ub"] + 1) / info["N_dft"]
         Fz = np.repeat(info["F_number_values"][:, None], pz.size, axis=1)
@@ -237,7 +248,8 @@ def das_pw(positions_x, positions_z, data_RF, f_s, steering_angle, element_width
             from .plan import PwPlan
 
             plan = PwPlan(px, pz, n_t, n_el, ang, f_s, w, p, c, fb, t0, window=win, F_number=F, normalization=nrm,
-                          index_gpu=pf.index, semantics=semantics, precision=precision, output_scale=output_scale)
+                          index_gpu=pf.index, semantics=semantics, precision=precision, output_scale=output_scale,
+                          virtual_sources=virtual_sources)
             image, sig = _execute_with_signal(plan, rf, px.size, pz.size, pf.index)
             return (image,) + outs + (sig,)
     rfF = np.asfortranarray(rf, dtype=np.float64 if rf.dtype == np.float64 else np.float32)

@@ -51,6 +51,8 @@ class Workload:
     # flow tube (x, z, radius, angle rad, v_max m/s) and the frame time (s)
     tube: tuple = (0.0, 0.0, 0.0, 0.0, 0.0)
     frame_time: float = 0.0
+    # virtual-source transmits (x_vs, z_vs) in m, one per entry of angles_deg (PW mode), or None
+    virtual_sources: np.ndarray | None = None
 
     @property
     def angles(self) -> np.ndarray:
@@ -76,7 +78,7 @@ class Workload:
         P.f_s, P.element_pitch, P.c_0 = self.f_s, self.pitch, self.c_0
         P.f_c, P.fractional_bandwidth = 5e6, 0.6
         P.snr_db = self.snr_db
-        P.mode = 0 if self.mode == "pw" else 1
+        P.mode = (2 if self.virtual_sources is not None else 0) if self.mode == "pw" else 1
         ang = np.ascontiguousarray(self.angles, dtype=np.float64)
         P.n_angles = ang.size if self.mode == "pw" else 0
         P.steering_angles = ang.ctypes.data_as(C.POINTER(C.c_double))
@@ -91,6 +93,11 @@ class Workload:
             keep.append(d)
         P.tube_x, P.tube_z, P.tube_radius, P.tube_angle, P.tube_v_max = (float(v) for v in self.tube)
         P.frame_time = float(self.frame_time)
+        if self.virtual_sources is not None:
+            vs = np.ascontiguousarray(np.asarray(self.virtual_sources, dtype=np.float64).reshape(-1, 2))
+            assert vs.shape[0] == ang.size
+            P.virtual_sources = vs.ctypes.data_as(C.POINTER(C.c_double))
+            keep.append(vs)
         return P, keep
 
     def rf_host(self) -> np.ndarray:
@@ -148,6 +155,16 @@ def saft_full() -> Workload:
     return Workload("saft_128x128_512x512", "sa", 3000, np.zeros(0), x, z, (2.25e6, 6.75e6), seed=20260002)
 
 
+def diverging_waves(n_sources: int = 11, z_vs: float = -15e-3) -> Workload:
+    """Diverging-wave compounding: n_sources virtual sources at z_vs behind the array,
+    x_vs spread over [-10, 10] mm, 2048 samples, 2.25-6.75 MHz, 256 x 256 voxels over
+    x in [-19, 19] mm, z in [5, 40] mm."""
+    w = Workload("diverging_11vs_256x256", "pw", 2048, np.zeros(n_sources), _lin(-19e-3, 19e-3, 256),
+                 _lin(5e-3, 40e-3, 256), (2.25e6, 6.75e6), seed=20260005)
+    w.virtual_sources = np.stack([np.linspace(-10e-3, 10e-3, n_sources), np.full(n_sources, z_vs)], axis=1)
+    return w
+
+
 def autofocus_layer() -> Workload:
     """Differentiation-form acquisition: the 11-angle array of cpwc_small (2048 samples,
     2.25-6.75 MHz) over a 6 mm surface layer at 1450 m/s on a 1540 m/s background, with
@@ -184,6 +201,8 @@ def scaled(w: Workload, n_x: int, n_z: int, n_angles: int | None = None, n_sampl
     if n_angles is not None and w.mode == "pw":
         idx = np.round(np.linspace(0, w.angles_deg.size - 1, n_angles)).astype(int)
         v.angles_deg = w.angles_deg[idx]
+        if w.virtual_sources is not None:
+            v.virtual_sources = np.asarray(w.virtual_sources)[idx]
     if n_samples is not None:
         v.n_samples = n_samples
     if n_speckle is not None:

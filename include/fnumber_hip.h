@@ -140,6 +140,14 @@ typedef struct fn_pw_params {
      * n_angles; RF [n_frames][n_angles][n_elements][n_samples], images
      * [n_frames][n_x][n_z].  0 or 1: one frame (the reference call). */
     int32_t n_frames;
+    /* virtual-source transmits (no counterpart in the reference; its
+     * gpu_bf_das_pw_rf.cu:63 lists diverging waves as to do): NULL for plane
+     * waves, else [n_angles][2] = (x_vs, z_vs) in m per transmit, replacing the
+     * steering angles (which are then not read).  z_vs < 0: diverging wave,
+     * transmit time (|r - r_vs| - |r_vs|) / c (zero where the wavefront crosses
+     * the array centre); z_vs > 0: focused wave, (|r_vs| +- |r - r_vs|) / c
+     * (+ below the focus).  Both plus index_t0 / f_s. */
+    const double *virtual_sources;
 } fn_pw_params;
 
 /* Full synthetic aperture (das_sa.m:1, gpu_bf_saft_rf.cu:107).
@@ -311,7 +319,7 @@ typedef struct fn_synth_params {
     double f_s, element_pitch, c_0;
     double f_c, fractional_bandwidth;  /* Gaussian-modulated pulse */
     double snr_db;                 /* white noise relative to signal RMS; +inf: none */
-    int32_t mode;                  /* 0: plane waves (angles), 1: synthetic aperture */
+    int32_t mode;                  /* 0: plane waves (angles), 1: synthetic aperture, 2: virtual sources */
     int32_t n_angles;
     const double *steering_angles;
     int32_t on_device;             /* 1: rf is a device pointer, computed on the GPU */
@@ -325,6 +333,9 @@ typedef struct fn_synth_params {
      * displaced by v * frame_time and wrapped over the lateral extent; R = 0: none */
     double tube_x, tube_z, tube_radius, tube_angle, tube_v_max;
     double frame_time;
+    /* mode 2: virtual-source transmits, [n_angles][2] = (x_vs, z_vs) in m, with the
+     * transmit times of fn_pw_params.virtual_sources (homogeneous medium) */
+    const double *virtual_sources;
 } fn_synth_params;
 /* rf: float32 column-major [n_samples x n_elements x (n_angles | n_elements)] */
 int32_t fn_synth_rf(const fn_synth_params *p, float *rf);

@@ -595,12 +595,21 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
             const double gr = want_vjp ? dx->g_re[v] : 0.0, gi = want_vjp ? dx->g_im[v] : 0.0;
             const double yr = want_adj ? dx->y_re[v] : 0.0, yi = want_adj ? dx->y_im[v] : 0.0;
             for (int ia = 0; ia < a->n_tx; ++ia) {
-                const double th = a->angles[ia];
-                const double ex = -sin(th), ez = cos(th);
-                const double sg = (ex > 0) - (ex < 0);
-                const double xref = sg * ((0 - M) * a->pitch);
-                /* das_pw.m:336, 339-340, 427-428 */
-                const double t_geo = (ex * (x - xref) + ez * z) / c_tof;
+                double t_geo;
+                if (a->virtual_sources) {
+                    /* virtual source: spherical wavefront from (x_vs, z_vs) */
+                    const double xv = a->virtual_sources[2 * ia], zv = a->virtual_sources[2 * ia + 1];
+                    const double r = sqrt((x - xv) * (x - xv) + (z - zv) * (z - zv));
+                    const double r0 = sqrt(xv * xv + zv * zv);
+                    t_geo = (zv < 0 ? r - r0 : r0 + (z >= zv ? r : -r)) / c_tof;
+                } else {
+                    const double th = a->angles[ia];
+                    const double ex = -sin(th), ez = cos(th);
+                    const double sg = (ex > 0) - (ex < 0);
+                    const double xref = sg * ((0 - M) * a->pitch);
+                    /* das_pw.m:336, 339-340, 427-428 */
+                    t_geo = (ex * (x - xref) + ez * z) / c_tof;
+                }
                 const double t_in = t_geo + a->index_t0 / a->f_s;
                 const double zc = z / c_tof;
                 for (int k = 0; k < n_el; ++k) {

@@ -91,6 +91,10 @@ typedef struct {
        voxel the CPU loop processes, [N_dft] complex (das_pw.m:477-481, das_sa.m:501-505),
        angles summed for PW */
     double *signal_re, *signal_im;
+    /* optional (PW): [n_tx][2] virtual sources (x_vs, z_vs) replacing the steering
+       angles: diverging (z_vs < 0) t_tx = (|r - r_vs| - |r_vs|) / c, focused
+       (z_vs > 0) t_tx = (|r_vs| + sign(z - z_vs) |r - r_vs|) / c */
+    const double *virtual_sources;
 } or_das_args_t;
 
 /* das_pw.m:302-474, compounded over the n_tx angles (examples.m:70-78). */

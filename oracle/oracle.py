@@ -59,6 +59,7 @@ class _Args(C.Structure):
         ("image_re", C.POINTER(C.c_double)), ("image_im", C.POINTER(C.c_double)),
         ("N_dft", C.c_int), ("n_lb", C.c_int), ("n_ub", C.c_int),
         ("signal_re", C.POINTER(C.c_double)), ("signal_im", C.POINTER(C.c_double)),
+        ("virtual_sources", C.POINTER(C.c_double)),
     ]
 
 
@@ -180,7 +181,7 @@ def _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics, spect
 def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
            window=("tukey", 0.2), F=None, normalization="on", semantics="cpu",
            spectra_mode=1, n_threads=0, sel=None, norm_window=("boxcar", 0.0),
-           c_tof=None, delay_offsets=None, want_signal=False):
+           c_tof=None, delay_offsets=None, want_signal=False, virtual_sources=None):
     """rf: [n_tx, n_el, n_t] C-order (MATLAB [n_t x n_el x n_tx]).  Returns (image, info).
 
     image is [n_z, n_x] complex (or [n_sel] with ``sel=(ix, iz)``), CPU scale.
@@ -200,6 +201,7 @@ def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
     a.norm_rx = NORM[normalization]
     a.norm_rx_win, a.norm_rx_win_param = WIN[norm_window[0]], float(norm_window[1])
     sig = _signal_buffers(a, keep, n_t, want_signal and sel is None)
+    _virtual_sources(a, keep, virtual_sources, n_tx)
     if c_tof is None:
         rc = lib().or_das_pw(C.byref(a))
     else:
@@ -217,6 +219,14 @@ def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
     if sig is not None:
         info["signal"] = (sig[0] + 1j * sig[1])[:a.N_dft]
     return img, info
+
+
+def _virtual_sources(a, keep, vs, n_tx):
+    a.virtual_sources = None
+    if vs is not None:
+        vs = np.ascontiguousarray(np.asarray(vs, dtype=np.float64).reshape(n_tx, 2))
+        a.virtual_sources = _dp(vs)
+        keep.append(vs)
 
 
 def _signal_buffers(a, keep, n_t, want):
@@ -244,7 +254,7 @@ class _Diff(C.Structure):
 def das_pw_diff(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
                 window=("tukey", 0.2), F=None, normalization="on", semantics="cpu",
                 spectra_mode=1, n_threads=0, norm_window=("boxcar", 0.0),
-                c_tof=None, delay_offsets=None, g=None, y=None, want_dc=False):
+                c_tof=None, delay_offsets=None, g=None, y=None, want_dc=False, virtual_sources=None):
     """FP64 derivatives of the PW image (oracle.h or_das_pw_diff).
 
     g, y: complex [n_z, n_x] (image layout).  Returns a dict with 'image',
@@ -265,6 +275,8 @@ def das_pw_diff(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0
     a.norm_rx_win, a.norm_rx_win_param = WIN[norm_window[0]], float(norm_window[1])
     nx, nz = np.asarray(x).size, np.asarray(z).size
     nv = nx * nz
+    a.signal_re = a.signal_im = None
+    _virtual_sources(a, keep, virtual_sources, n_tx)
     d = _Diff()
     out = {}
 

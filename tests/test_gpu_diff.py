@@ -209,3 +209,34 @@ def test_set_tof_default_restores_plain_image(fa, torch):
     plan.close()
     assert (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a)).item() < 1e-6
     assert (torch.linalg.vector_norm(a - c) / torch.linalg.vector_norm(a)).item() > 1e-2
+
+
+def test_virtual_source_tangent_and_adjoint_match_oracle(fa, oracle, torch):
+    """The differentiation form with diverging / focused transmits (transmit time
+    (|r - r_vs| -+ |r_vs|) / c scales as 1 / c like the plane wave's)."""
+    w, rf, delays = case(64, 3)
+    vs = np.array([[-3e-3, -10e-3], [1e-3, -20e-3], [0.0, 15e-3]])
+    c = 1508.0
+    print("case virtual-source tangent/adjoint", flush=True)
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     w.c_0, w.f_bounds, 0.0, window=fa.windows.tukey(0.5),
+                     F_number=fa.f_numbers.grating.angle_lb(60, 1.5), semantics="cpu", output_scale="cpu",
+                     virtual_sources=vs)
+    plan.set_tof(c, delays)
+    rfd = torch.from_numpy(rf).cuda()
+    n = (w.positions_x.size, w.positions_z.size)
+    img = torch.empty(n, dtype=torch.complex64, device="cuda")
+    dimg = torch.empty_like(img)
+    plan.execute_tangent(rfd, img, dimg)
+    rng = np.random.default_rng(3)
+    y = rng.normal(size=(w.positions_z.size, w.positions_x.size)) + 1j * rng.normal(size=(w.positions_z.size,
+                                                                                          w.positions_x.size))
+    adj = plan.adjoint(_img_dev(torch, y)).cpu().numpy()
+    torch.cuda.synchronize()
+    ref = oracle.das_pw_diff(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0, w.f_bounds,
+                             0.0, **_okw(oracle, "tukey", "cpu", "on"), c_tof=c, delay_offsets=delays, y=y,
+                             want_dc=True, virtual_sources=vs)
+    e = (rel_l2(_img_np(img), ref["image"]), rel_l2(_img_np(dimg), ref["dIdc"]), rel_l2(adj, ref["adjoint"]))
+    print(f"  image {e[0]:.2e} dI/dc {e[1]:.2e} adjoint {e[2]:.2e}", flush=True)
+    plan.close()
+    assert max(e) <= 1e-4

@@ -282,3 +282,51 @@ def test_das_pw_nargout_matches_cpu_outputs(fa, oracle, small_pw):
                               w.f_bounds, 0, ("tukey", 0.5), oracle.fnumber("angle_lb", 60, 1.5, 10), "on",
                               want_signal=True)
     assert rel_l2(sig, info["signal"]) <= TOL_L2
+
+
+# ---- virtual-source transmits (diverging and focused waves) ------------------
+VS = np.array([[-4e-3, -10e-3], [0.0, -15e-3], [5e-3, -8e-3], [0.0, 20e-3], [2e-3, 9e-3]])
+
+
+@pytest.mark.parametrize("win,sem", [("tukey", "cpu"), ("hann", "cpu"), ("boxcar", "mex")])
+def test_das_pw_virtual_sources_match_oracle(fa, oracle, small_pw, win, sem):
+    w, _ = small_pw
+    rng = np.random.default_rng(31)
+    rf = rng.standard_normal((VS.shape[0], w.n_elements, w.n_samples)).astype(np.float32)
+    Fo = fa.f_numbers.grating.angle_lb(60, 1.5)
+    print(f"case virtual sources win={win} sem={sem}", flush=True)
+    img = fa.das_pw(w.positions_x, w.positions_z, np.transpose(rf, (2, 1, 0)), w.f_s, np.zeros(VS.shape[0]), w.width,
+                    w.pitch, w.c_0, w.f_bounds, 0, _win(fa, win), Fo, fa.normalizations.on(), fa.platforms.gpu(0),
+                    semantics=sem, output_scale="cpu", virtual_sources=VS)
+    ref, _ = oracle.das_pw(w.positions_x, w.positions_z, rf, w.f_s, np.zeros(VS.shape[0]), w.width, w.pitch, w.c_0,
+                           w.f_bounds, 0, (win, 0.5 if win == "tukey" else 0.0), _orc_F(oracle, Fo), "on",
+                           semantics=sem, virtual_sources=VS)
+    e, d = rel_l2(img, ref), peak_db(img, ref)
+    print(f"  rel_l2={e:.3e} peak={d:.1f} dB", flush=True)
+    assert e <= TOL_L2 and d <= TOL_DB
+
+
+def test_virtual_source_synth_focuses_points(fa):
+    """Synthesizer (mode 2) and beamformer share the virtual-source model: point
+    scatterers of a diverging-wave acquisition image at their positions."""
+    import torch
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.diverging_waves(), 96, 96, n_angles=5, n_speckle=0)
+    w.n_points, w.snr_db = 6, np.inf
+    xs, zs, _ = w.scatterers()
+    print("case virtual-source synth", flush=True)
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     w.c_0, w.f_bounds, 0.0, window=fa.windows.hann(), virtual_sources=w.virtual_sources)
+    img = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device="cuda")
+    plan.execute(w.rf_device("cuda:0"), img)
+    a = img.abs().cpu().numpy()
+    plan.close()
+    dx, dz = np.diff(w.positions_x[:2])[0], np.diff(w.positions_z[:2])[0]
+    for xp, zp in zip(xs, zs):
+        ix, iz = int(round((xp - w.positions_x[0]) / dx)), int(round((zp - w.positions_z[0]) / dz))
+        if not (3 <= ix < a.shape[0] - 3 and 3 <= iz < a.shape[1] - 3):
+            continue
+        win = a[ix - 3:ix + 4, iz - 3:iz + 4]
+        jx, jz = np.unravel_index(np.argmax(win), win.shape)
+        assert abs(jx - 3) <= 1 and abs(jz - 3) <= 1, (xp, zp)

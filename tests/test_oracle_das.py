@@ -207,3 +207,52 @@ def test_oracle_windowed_equals_numpy_transliteration(oracle, win, taper):
         # the taper changes the image (it is applied)
         on, _ = oracle.das_pw(x, z, rf.T[None], FS, [ang], W, P, C0, (2.25e6, 6.75e6), t0, win, F, "on")
         assert np.linalg.norm(img[fin] - on[fin]) > 1e-3 * np.linalg.norm(on[fin])
+
+
+def test_oracle_virtual_source_far_limit(oracle):
+    """A virtual source far behind the array on the axis is a plane wave at 0 deg:
+    (|r - r_vs| - |r_vs|) -> z as z_vs -> -inf (error ~ x^2 / (2 |z_vs|))."""
+    rng = np.random.default_rng(21)
+    x = np.linspace(-3e-3, 3e-3, 6)
+    z = np.linspace(5e-3, 12e-3, 7)
+    rf = rng.standard_normal((1, 16, 256)).astype(np.float32)
+    F = oracle.fnumber("angle_lb", 60, 1.5, 10)
+    pw, _ = oracle.das_pw(x, z, rf, FS, [0.0], W, P, C0, (2.25e6, 6.75e6), 0, ("tukey", 0.5), F, "on")
+    vs, _ = oracle.das_pw(x, z, rf, FS, [0.0], W, P, C0, (2.25e6, 6.75e6), 0, ("tukey", 0.5), F, "on",
+                          virtual_sources=[[0.0, -1e5]])
+    assert np.linalg.norm(vs - pw) <= 1e-5 * np.linalg.norm(pw)
+    near, _ = oracle.das_pw(x, z, rf, FS, [0.0], W, P, C0, (2.25e6, 6.75e6), 0, ("tukey", 0.5), F, "on",
+                            virtual_sources=[[0.0, -5e-3]])
+    assert np.linalg.norm(near - pw) > 1e-2 * np.linalg.norm(pw)  # a near source is not a plane wave
+
+
+def _echo_rf(xp, zp, t_tx, n_el=32, n_t=1024, fc=5e6, bw=0.6):
+    """RF [1, n_el, n_t] of one point scatterer, the echo a Gaussian-modulated pulse at
+    t_tx + |r_k - r_p| / c (numpy, independent of the product's synthesizer)."""
+    M = (n_el - 1) / 2
+    xk = (np.arange(n_el) - M) * P
+    tau = t_tx + np.sqrt((xk - xp) ** 2 + zp ** 2) / C0
+    t = np.arange(n_t) / FS
+    sig_f = bw * fc / (2 * np.sqrt(2 * np.log(2)))
+    s = 1 / (2 * np.pi * sig_f)
+    d = t[None, :] - tau[:, None]
+    return (np.exp(-d ** 2 / (2 * s ** 2)) * np.cos(2 * np.pi * fc * d))[None].astype(np.float32)
+
+
+@pytest.mark.parametrize("xv,zv,zp", [(2e-3, -8e-3, 14e-3), (-1e-3, 12e-3, 18e-3), (0.0, 16e-3, 10e-3)])
+def test_oracle_virtual_source_point_target(oracle, xv, zv, zp):
+    """Diverging (z_vs < 0) and focused (z_vs > 0, the point below and above the focus)
+    transmits: the image of a point scatterer peaks at the scatterer with the pulse's
+    zero phase (all echoes aligned by the transmit + receive times of the model)."""
+    xp = 1.2e-3
+    r0 = np.hypot(xv, zv)
+    r = np.hypot(xp - xv, zp - zv)
+    t_tx = ((r - r0) if zv < 0 else (r0 + np.sign(zp - zv) * r)) / C0
+    rf = _echo_rf(xp, zp, t_tx)
+    x = xp + np.arange(-6, 7) * 0.1e-3
+    z = zp + np.arange(-6, 7) * 0.05e-3
+    img, _ = oracle.das_pw(x, z, rf, FS, [0.0], W, P, C0, (2.25e6, 7.75e6), 0, ("hann", 0.0),
+                           oracle.fnumber("constant", 1.0), "on", virtual_sources=[[xv, zv]])
+    iz, ix = np.unravel_index(np.argmax(np.abs(img)), img.shape)
+    assert (iz, ix) == (6, 6)
+    assert abs(np.angle(img[6, 6])) < 0.1

@@ -131,6 +131,7 @@ PROTOTYPES = {
     "fn_batch_execute": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_void_p), C.c_void_p]),
     "fn_batch_info": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "fn_batch_set_persistent": (C.c_int32, [C.c_void_p, C.c_int32]),
     "fn_batch_destroy": (C.c_int32, [C.c_void_p]),
     "fn_plan_focused_signal": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fn_plan_set_tof": (C.c_int32, [C.c_void_p, C.c_double, _dp]),

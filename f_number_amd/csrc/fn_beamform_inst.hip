@@ -169,6 +169,64 @@ hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, 
 }
 #endif
 
+// persistent batch launches: one kernel per class over the work items of many
+// plans (grid <= 0: only report the resident workgroups per CU in *occupancy)
+#define FN_PERSIST_LAUNCHER(m, k, p) FN_CAT6(launch_beamform_persistent_m, m, _k, k, _p, p)
+template <int MT, int WCR, int WCT>
+static hipError_t launch_persist_one(int grid, size_t lds, hipStream_t s, const BeamArgs *jobs, const int4 *items,
+                                     int n_items, int *counter, int *occupancy)
+{
+    auto kern = beamform_persistent_kernel<INST_MODE, MT, INST_KI, WCR, WCT, INST_PREC>;
+    static size_t lds_done = 0;
+    if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
+    if (grid <= 0) {
+        int n = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 256, lds);
+        *occupancy = n;
+        return e;
+    }
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, counter);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), lds, s, jobs, items, n_items, counter);
+    return hipGetLastError();
+}
+
+template <int MT, int WCT>
+static hipError_t launch_persist_wcr(int wcr, int grid, size_t lds, hipStream_t s, const BeamArgs *jobs,
+                                     const int4 *items, int n_items, int *counter, int *occ)
+{
+    switch (wcr) {
+    case WC_RANGE: return launch_persist_one<MT, WC_RANGE, WCT>(grid, lds, s, jobs, items, n_items, counter, occ);
+    case WC_LINEAR: return launch_persist_one<MT, WC_LINEAR, WCT>(grid, lds, s, jobs, items, n_items, counter, occ);
+    case WC_COS_ALL: return launch_persist_one<MT, WC_COS_ALL, WCT>(grid, lds, s, jobs, items, n_items, counter, occ);
+    case WC_COS_RF: return launch_persist_one<MT, WC_COS_RF, WCT>(grid, lds, s, jobs, items, n_items, counter, occ);
+    default: return launch_persist_one<MT, WC_COS, WCT>(grid, lds, s, jobs, items, n_items, counter, occ);
+    }
+}
+
+hipError_t FN_PERSIST_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, int grid, size_t lds,
+                                                            hipStream_t s, const BeamArgs *jobs, const int4 *items,
+                                                            int n_items, int *counter, int *occupancy)
+{
+#if INST_MODE == 0
+    (void)wct;
+    switch (mt) {
+    case 1: return launch_persist_wcr<1, WC_RANGE>(wcr, grid, lds, s, jobs, items, n_items, counter, occupancy);
+    case 2: return launch_persist_wcr<2, WC_RANGE>(wcr, grid, lds, s, jobs, items, n_items, counter, occupancy);
+    case 3: return launch_persist_wcr<3, WC_RANGE>(wcr, grid, lds, s, jobs, items, n_items, counter, occupancy);
+    case 4: return launch_persist_wcr<4, WC_RANGE>(wcr, grid, lds, s, jobs, items, n_items, counter, occupancy);
+    case 5: return launch_persist_wcr<5, WC_RANGE>(wcr, grid, lds, s, jobs, items, n_items, counter, occupancy);
+    default: return hipErrorInvalidValue;
+    }
+#else
+    if (mt != INST_KI / 4) return hipErrorInvalidValue;
+    switch (wct) {
+    case WC_RANGE: return launch_persist_wcr<INST_KI / 4, WC_RANGE>(wcr, grid, lds, s, jobs, items, n_items, counter, occupancy);
+    case WC_LINEAR: return launch_persist_wcr<INST_KI / 4, WC_LINEAR>(wcr, grid, lds, s, jobs, items, n_items, counter, occupancy);
+    default: return launch_persist_wcr<INST_KI / 4, WC_COS>(wcr, grid, lds, s, jobs, items, n_items, counter, occupancy);
+    }
+#endif
+}
+
 #if INST_PREC == 0
 // windowed-normalization variant (FP32, general window classes): plane-wave
 // groups of at most two transmit tiles, synthetic aperture at its fixed tiling

@@ -49,6 +49,25 @@ hipError_t launch_beamform_nw_m0_k32(int, int, int, dim3, size_t, hipStream_t, c
 hipError_t launch_beamform_nw_m1_k8(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_nw_m1_k16(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_nw_m1_k32(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+#define FN_DECL_PERSIST(m, k)                                                                                \
+    hipError_t launch_beamform_persistent_m##m##_k##k##_p0(int, int, int, int, size_t, hipStream_t,               \
+                                                           const BeamArgs *, const int4 *, int, int *, int *);    \
+    hipError_t launch_beamform_persistent_m##m##_k##k##_p1(int, int, int, int, size_t, hipStream_t,               \
+                                                           const BeamArgs *, const int4 *, int, int *, int *);    \
+    hipError_t launch_beamform_persistent_m##m##_k##k##_p2(int, int, int, int, size_t, hipStream_t,               \
+                                                           const BeamArgs *, const int4 *, int, int *, int *);    \
+    hipError_t launch_beamform_persistent_m##m##_k##k##_p3(int, int, int, int, size_t, hipStream_t,               \
+                                                           const BeamArgs *, const int4 *, int, int *, int *);
+FN_DECL_PERSIST(0, 8) FN_DECL_PERSIST(0, 16) FN_DECL_PERSIST(0, 32)
+FN_DECL_PERSIST(1, 8) FN_DECL_PERSIST(1, 16) FN_DECL_PERSIST(1, 32)
+typedef hipError_t (*PersistLauncher)(int, int, int, int, size_t, hipStream_t, const BeamArgs *, const int4 *, int,
+                                      int *, int *);
+#define FN_PERSIST_ROW(m, k)                                                                                 \
+    {launch_beamform_persistent_m##m##_k##k##_p0, launch_beamform_persistent_m##m##_k##k##_p1,                 \
+     launch_beamform_persistent_m##m##_k##k##_p2, launch_beamform_persistent_m##m##_k##k##_p3}
+static const PersistLauncher kPersistLaunchers[2][3][4] = {
+    {FN_PERSIST_ROW(0, 8), FN_PERSIST_ROW(0, 16), FN_PERSIST_ROW(0, 32)},
+    {FN_PERSIST_ROW(1, 8), FN_PERSIST_ROW(1, 16), FN_PERSIST_ROW(1, 32)}};
 hipError_t launch_windowed_rows(const BeamArgs &A, int mode, int win_rx, double par_rx, float *h_rx, int win_tx,
                                 double par_tx, float *h_tx, hipStream_t s);
 hipError_t launch_focused_signal(const BeamArgs &A, int mode, int wcr, int wct, int MT_all, int KI, int nw, int ix,
@@ -484,12 +503,9 @@ int fill_zero(float *p, long n, hipStream_t s)
     return FN_OK;
 }
 
-int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w, float *d_w_rx, hipStream_t user)
+// the beamforming arguments of one execute (image and weight buffers)
+fnk::BeamArgs beam_args(const fn_plan *P, float *d_image, float *d_w, float *d_w_rx)
 {
-    if (!P || !d_rf || !d_image) return fail(FN_ERR_ARGUMENT, "null argument");
-    if (dtype != FN_DTYPE_F32 && dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
-    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
-    hipStream_t s = user;  // NULL: the default stream
     fnk::BeamArgs A = P->args;
     A.image = reinterpret_cast<float2 *>(d_image);
     if (P->mode == 0) {
@@ -499,23 +515,18 @@ int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w,
         A.weights = d_w_rx;
         A.weights_tx = d_w;
     }
-    Run run{};
-    if (P->profiling) {
-        if (!P->spare.empty()) {
-            run = P->spare.back();
-            P->spare.pop_back();
-        } else {
-            HIP_CHECK(hipEventCreate(&run.e0), "hipEventCreate");
-            HIP_CHECK(hipEventCreate(&run.e1), "hipEventCreate");
-            HIP_CHECK(hipEventCreate(&run.e2), "hipEventCreate");
-        }
-    }
-    if (A.atomic_out || P->n_frames > 1) {
+    return A;
+}
+
+// everything of an execute before the beamforming launch: the zeroed image for
+// atomic accumulation, the band spectra and (16-bit forms) their planes
+int prepare(fn_plan *P, const void *d_rf, int dtype, float *d_image, hipStream_t s)
+{
+    if (P->args.atomic_out || P->n_frames > 1) {
         const long n = 2 * P->n_vox * P->n_frames;
         hipLaunchKernelGGL(fnk::fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_image, n);
         HIP_CHECK(hipGetLastError(), "fill_zero_kernel");
     }
-    if (P->profiling) HIP_CHECK(hipEventRecord(run.e0, s), "hipEventRecord");
     const fnk::SpecArgs &S = P->sargs;
     if (int st = launch_spectra(P, d_rf, dtype, s)) return st;
     if (P->precision != FN_PRECISION_FP32) {
@@ -533,6 +544,29 @@ int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w,
         }
         HIP_CHECK(hipGetLastError(), "lowp_planes_kernel");
     }
+    return FN_OK;
+}
+
+int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w, float *d_w_rx, hipStream_t user)
+{
+    if (!P || !d_rf || !d_image) return fail(FN_ERR_ARGUMENT, "null argument");
+    if (dtype != FN_DTYPE_F32 && dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
+    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
+    hipStream_t s = user;  // NULL: the default stream
+    const fnk::BeamArgs A = beam_args(P, d_image, d_w, d_w_rx);
+    Run run{};
+    if (P->profiling) {
+        if (!P->spare.empty()) {
+            run = P->spare.back();
+            P->spare.pop_back();
+        } else {
+            HIP_CHECK(hipEventCreate(&run.e0), "hipEventCreate");
+            HIP_CHECK(hipEventCreate(&run.e1), "hipEventCreate");
+            HIP_CHECK(hipEventCreate(&run.e2), "hipEventCreate");
+        }
+    }
+    if (P->profiling) HIP_CHECK(hipEventRecord(run.e0, s), "hipEventRecord");
+    if (int st = prepare(P, d_rf, dtype, d_image, s)) return st;
     if (P->profiling) HIP_CHECK(hipEventRecord(run.e1, s), "hipEventRecord");
     const long tiles = (P->n_vox + 63) / 64;
     hipError_t e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups), A, s);
@@ -1147,9 +1181,23 @@ int32_t fn_das_sa(const fn_sa_params *p, const void *data_rf, int32_t rf_dtype, 
 
 // ---- batches: many plans captured into one graph ----------------------
 
+// plans of one kernel class (mode, element steps, transmit tiles, window classes,
+// precision) beamformed by one persistent launch over their work items
+struct BatchClass {
+    std::vector<int> plans;
+    fnk::BeamArgs *d_jobs = nullptr;  // one BeamArgs per plan (its buffers)
+    int4 *d_items = nullptr;          // (job, voxel tile, bin chunk, transmit group), longest chunks first
+    int *d_counter = nullptr;
+    int n_items = 0, grid = 0;
+};
+
 struct fn_batch {
     std::vector<fn_plan *> plans;
     int device = 0;
+    int persistent = 1;                // plans sharing a kernel class run as one persistent launch
+    std::vector<BatchClass> classes;
+    std::vector<int> singles;          // plans launched on their own
+    hipStream_t pstream = nullptr;     // stream of the persistent launches inside the capture
     std::vector<hipStream_t> streams;  // forked worker streams (default priority)
     hipStream_t capture = nullptr;
     std::vector<hipEvent_t> joins;
@@ -1169,6 +1217,86 @@ static void batch_release_graph(fn_batch *b)
     if (b->graph) hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+    for (auto &c : b->classes) {
+        hipFree(c.d_jobs);
+        hipFree(c.d_items);
+        hipFree(c.d_counter);
+    }
+    b->classes.clear();
+    b->singles.clear();
+}
+
+// kernel class of a plan for the persistent launches; -1: launched on its own
+// (ensembles, windowed normalization)
+static long batch_class_key(const fn_plan *P)
+{
+    if (P->n_frames > 1 || P->nw) return -1;
+    return ((((long)P->mode * 64 + P->KI) * 16 + P->MT_tmpl) * 8 + P->wcr) * 8 * 4 + P->wct * 4 + P->precision;
+}
+
+// group the plans into classes, upload each class's job and work-item tables and
+// size its grid (resident workgroups x CUs); outside any capture
+static int batch_plan_classes(fn_batch *b, float *const *d_images)
+{
+    std::vector<std::pair<long, std::vector<int>>> groups;
+    for (int i = 0; i < (int)b->plans.size(); ++i) {
+        const long key = b->persistent ? batch_class_key(b->plans[i]) : -1;
+        if (key < 0) { b->singles.push_back(i); continue; }
+        auto it = std::find_if(groups.begin(), groups.end(), [&](const auto &g) { return g.first == key; });
+        if (it == groups.end()) groups.push_back({key, {i}});
+        else it->second.push_back(i);
+    }
+    // a class launch waits for the spectra of all its plans: classes are cut into
+    // launches of at most kPersistPlans plans, so that one launch beamforms while
+    // the worker streams prepare the next
+    constexpr size_t kPersistPlans = 8;
+    std::vector<std::vector<int>> launches;
+    for (auto &g : groups) {
+        if (g.second.size() < 2) { b->singles.push_back(g.second[0]); continue; }
+        const size_t n_l = (g.second.size() + kPersistPlans - 1) / kPersistPlans;
+        for (size_t l = 0; l < n_l; ++l) {
+            const size_t lo = g.second.size() * l / n_l, hi = g.second.size() * (l + 1) / n_l;
+            launches.emplace_back(g.second.begin() + lo, g.second.begin() + hi);
+        }
+    }
+    int n_cu = 256;
+    hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device);
+    for (auto &plans : launches) {
+        BatchClass c;
+        c.plans = plans;
+        std::vector<fnk::BeamArgs> jobs;
+        std::vector<std::pair<int, int4>> items;  // (bins of the chunk, item)
+        for (int j = 0; j < (int)c.plans.size(); ++j) {
+            const fn_plan *P = b->plans[c.plans[j]];
+            jobs.push_back(beam_args(P, d_images[c.plans[j]], nullptr, nullptr));
+            const int tiles = (int)((P->n_vox + 63) / 64), chunk = P->args.bin_chunk;
+            for (int bz = 0; bz < P->n_groups; ++bz)
+                for (int by = 0; by < P->bin_split; ++by) {
+                    const int bins = std::min(chunk, P->g.n_f - by * chunk);
+                    for (int bx = 0; bx < tiles; ++bx) items.push_back({bins, make_int4(j, bx, by, bz)});
+                }
+        }
+        std::stable_sort(items.begin(), items.end(), [](const auto &a, const auto &b2) { return a.first > b2.first; });
+        std::vector<int4> it4(items.size());
+        for (size_t q = 0; q < items.size(); ++q) it4[q] = items[q].second;
+        c.n_items = (int)it4.size();
+        const fn_plan *P0 = b->plans[c.plans[0]];
+        const int kidx = P0->KI == 8 ? 0 : (P0->KI == 16 ? 1 : 2);
+        int occ = 0;
+        hipError_t e = fnk::kPersistLaunchers[P0->mode][kidx][P0->precision](P0->MT_tmpl, P0->wcr, P0->wct, 0,
+                                                                            P0->lds_beam, nullptr, nullptr, nullptr, 0,
+                                                                            nullptr, &occ);
+        if (e != hipSuccess) return hip_fail(e, "persistent kernel occupancy");
+        c.grid = std::max(1, std::min(c.n_items, std::max(1, occ) * n_cu));
+        e = hipMalloc(&c.d_jobs, jobs.size() * sizeof(fnk::BeamArgs));
+        if (e == hipSuccess) e = hipMalloc(&c.d_items, it4.size() * sizeof(int4));
+        if (e == hipSuccess) e = hipMalloc(&c.d_counter, sizeof(int));
+        if (e == hipSuccess) e = hipMemcpy(c.d_jobs, jobs.data(), jobs.size() * sizeof(fnk::BeamArgs), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c.d_items, it4.data(), it4.size() * sizeof(int4), hipMemcpyHostToDevice);
+        b->classes.push_back(c);  // freed with the graph, also on failure
+        if (e != hipSuccess) return hip_fail(e, "persistent batch tables");
+    }
+    return FN_OK;
 }
 
 int32_t fn_batch_create(fn_plan *const *plans, int32_t n_plans, int32_t n_streams, fn_batch **out)
@@ -1188,6 +1316,7 @@ int32_t fn_batch_create(fn_plan *const *plans, int32_t n_plans, int32_t n_stream
     int st = FN_OK;
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->pstream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     for (int i = 0; i < n_streams && e == hipSuccess; ++i) {
         hipStream_t w = nullptr;
@@ -1228,20 +1357,57 @@ int32_t fn_batch_execute(fn_batch *b, const void *const *d_rf, const int32_t *dt
             if (st) return st;
         }
         HIP_CHECK(hipStreamSynchronize(s), "batch warm-up");
+        if (int st = batch_plan_classes(b, d_images)) { batch_release_graph(b); return st; }
         HIP_CHECK(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         int st = FN_OK;
+        const size_t W = b->streams.size();
         hipError_t e = hipEventRecord(b->fork, b->capture);
-        for (size_t w = 0; w < b->streams.size() && e == hipSuccess; ++w) e = hipStreamWaitEvent(b->streams[w], b->fork, 0);
-        for (size_t i = 0; i < n && e == hipSuccess && !st; ++i) {
-            const int prof = b->plans[i]->profiling;
-            b->plans[i]->profiling = 0;
-            st = execute(b->plans[i], d_rf[i], dtypes[i], d_images[i], nullptr, nullptr,
-                         b->streams[i % b->streams.size()]);
-            b->plans[i]->profiling = prof;
+        for (size_t w = 0; w < W && e == hipSuccess; ++w) e = hipStreamWaitEvent(b->streams[w], b->fork, 0);
+        // plans on their own (ensembles and the like, usually the longest) first, one
+        // worker each; the persistent classes' spectra on the other workers
+        const size_t n_single = b->singles.size();
+        size_t slot = 0;
+        for (int i : b->singles) {
+            if (e != hipSuccess || st) break;
+            fn_plan *P = b->plans[i];
+            hipStream_t ws = b->streams[slot++ % W];
+            st = prepare(P, d_rf[i], dtypes[i], d_images[i], ws);
+            const long tiles = (P->n_vox + 63) / 64;
+            if (!st)
+                e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups),
+                                beam_args(P, d_images[i], nullptr, nullptr), ws);
         }
-        for (size_t w = 0; w < b->streams.size() && e == hipSuccess && !st; ++w) {
+        const size_t w0 = n_single < W ? n_single : 0, n_w = n_single < W ? W - n_single : W;
+        slot = 0;
+        for (auto &c : b->classes) {
+            if (e != hipSuccess || st) break;
+            std::vector<char> used(W, 0);
+            for (int i : c.plans) {
+                if (st) break;
+                const size_t w = w0 + slot++ % n_w;
+                st = prepare(b->plans[i], d_rf[i], dtypes[i], d_images[i], b->streams[w]);
+                used[w] = 1;
+            }
+            // the class's launch waits for exactly its own plans' spectra
+            for (size_t w = 0; w < W && e == hipSuccess && !st; ++w) {
+                if (!used[w]) continue;
+                e = hipEventRecord(b->joins[w], b->streams[w]);
+                if (e == hipSuccess) e = hipStreamWaitEvent(b->pstream, b->joins[w], 0);
+            }
+            if (e != hipSuccess || st) break;
+            const fn_plan *P0 = b->plans[c.plans[0]];
+            const int kidx = P0->KI == 8 ? 0 : (P0->KI == 16 ? 1 : 2);
+            e = fnk::kPersistLaunchers[P0->mode][kidx][P0->precision](P0->MT_tmpl, P0->wcr, P0->wct, c.grid,
+                                                                     P0->lds_beam, b->pstream, c.d_jobs, c.d_items,
+                                                                     c.n_items, c.d_counter, nullptr);
+        }
+        for (size_t w = 0; w < W && e == hipSuccess && !st; ++w) {
             e = hipEventRecord(b->joins[w], b->streams[w]);
             if (e == hipSuccess) e = hipStreamWaitEvent(b->capture, b->joins[w], 0);
+        }
+        if (!b->classes.empty() && e == hipSuccess && !st) {
+            e = hipEventRecord(b->fork, b->pstream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->capture, b->fork, 0);
         }
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(b->capture, &g);
@@ -1276,12 +1442,26 @@ int32_t fn_batch_destroy(fn_batch *b)
     hipSetDevice(b->device);
     for (hipStream_t w : b->streams) hipStreamSynchronize(w);
     if (b->capture) hipStreamSynchronize(b->capture);
+    if (b->pstream) hipStreamSynchronize(b->pstream);
     batch_release_graph(b);
     for (hipStream_t w : b->streams) hipStreamDestroy(w);
     for (hipEvent_t j : b->joins) hipEventDestroy(j);
     if (b->fork) hipEventDestroy(b->fork);
     if (b->capture) hipStreamDestroy(b->capture);
+    if (b->pstream) hipStreamDestroy(b->pstream);
     delete b;
+    return FN_OK;
+}
+
+int32_t fn_batch_set_persistent(fn_batch *b, int32_t enable)
+{
+    fnh::clear_error();
+    if (!b) return fail(FN_ERR_ARGUMENT, "null batch");
+    if (b->persistent != (enable != 0)) {
+        hipSetDevice(b->device);
+        batch_release_graph(b);  // captured again at the next execute
+    }
+    b->persistent = enable != 0;
     return FN_OK;
 }
 

@@ -426,6 +426,18 @@ struct WavesPerEU {
     static constexpr int value = (MT <= 2) ? 1 : (bytes <= 81920 ? 2 : 1);
 };
 
+// Occupancy target of the persistent batch kernel.  The claim loop around the
+// tile costs registers; left alone the compiler settles small plane-wave tiles
+// (MT x KI <= 64, where LDS admits 3+ workgroups per CU) at 2 waves per SIMD
+// where the one-launch-per-plan kernel reaches 3.  A target of 3 is stated where
+// the tile still fits without spilling: the 16-bit forms except the LINEAR and
+// general COS windows, and FP32 with the full-array Hann (WC_COS_ALL).
+template <int MODE, int MT, int KI, int WCR, int PREC>
+struct PersistWavesPerEU {
+    static constexpr bool fits3 = (PREC == PREC_FP32) ? (WCR == WC_COS_ALL) : (WCR != WC_LINEAR && WCR != WC_COS);
+    static constexpr int value = (MODE == MODE_PW && MT * KI <= 64 && fits3) ? 3 : WavesPerEU<MT, KI>::value;
+};
+
 // TAN (plane waves, FP32): also accumulates the tangent dI/dc_tof.  Geometric
 // delays scale as 1/c, so d/dc e^{j w tau} = j w (-(tau_tx,g + tau_rx,g) / c) e^{j w tau}:
 // a second contraction with B' = B tau_rx,g and the row factor tau_tx,g.
@@ -434,25 +446,28 @@ struct WavesPerEU {
 // bins and is added to its frame's image at the end (frame = row / rows_per_frame).
 // NW: windowed normalization (FP32, one bin chunk per block so that every lane
 // counts its voxel's valid rows from the first bin).
+// One workgroup's share of a plan: voxel tile bx (64 voxels), bin chunk by,
+// transmit group bz.  beamform_kernel runs it for its block index; the
+// persistent batch kernel for every work item it claims.
 template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<MT, KI>::value)))
-void beamform_kernel(BeamArgs A)
+__device__ __forceinline__ void beamform_tile(const BeamArgs &A, const int bx, const int by, const int bz,
+                                              const int tid)
 {
     static_assert(!TAN || (MODE == MODE_PW && PREC == PREC_FP32), "tangent: plane waves, FP32");
     static_assert(!ENS || (MODE == MODE_PW && !TAN), "ensemble: plane waves");
     static_assert(!NW || (PREC == PREC_FP32 && !TAN && !ENS), "windowed normalization: FP32 forward");
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, c = lane & 15;
     const long n_vox = (long)A.n_x * A.n_z;
-    const long v = (long)blockIdx.x * 64 + wave * 16 + c;
+    const long v = (long)bx * 64 + wave * 16 + c;
     const bool vok = v < n_vox;
     const long vv = vok ? v : n_vox - 1;
     const int ix = (int)(vv / A.n_z), iz = (int)(vv - (long)ix * A.n_z);
     const double x = A.px[ix], z = A.pz[iz];
     const double xp = x / A.pitch, zp = z / (2.0 * A.pitch);
     const double clamp_z = sqrt(z / (2.88 * A.c0));  // sqrt(f z / (2.88 c)) = sqrt(f) clamp_z (das_pw.m:374)
-    const int mt0 = blockIdx.z * A.tx_group;  // first M-tile of this transmit group
+    const int mt0 = bz * A.tx_group;  // first M-tile of this transmit group
 
     // receive phase increments: tau_rx = sqrt(((x_k-x)/c)^2 + (z/c)^2) (das_pw.m:339-340,427)
     // plus the element's delay offset
@@ -510,7 +525,7 @@ void beamform_kernel(BeamArgs A)
     // w_n = n * 2 pi f_s / N; the tangent factor -w_n / c_tof is applied per bin
     const float w_unit = (float)(2.0 * 3.141592653589793 * A.f_s / A.N);
     const float inv_c = (float)(1.0 / A.ctof);
-    const int b_begin = blockIdx.y * A.bin_chunk;
+    const int b_begin = by * A.bin_chunk;
     const int b_end = min(b_begin + A.bin_chunk, A.n_f);
     const int MT_all = (A.n_tx + 15) >> 4;  // stride of the operand layout
     // staged bytes per bin: this group's tiles of every plane
@@ -526,13 +541,13 @@ void beamform_kernel(BeamArgs A)
             if (PREC == PREC_FP32) {
                 const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)b * MT_all + mt0) * (tile_bytes / 16);
                 const int n4 = mt_here * (tile_bytes / 16);
-                for (int e = threadIdx.x; e < n_f4; e += 256) dst[e] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int e = tid; e < n_f4; e += 256) dst[e] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
             } else {
                 // planes [p][mt][c][64][8] per bin; copy tiles mt0.. of each plane
                 constexpr int pl = Prec<PREC>::planes;
                 constexpr int tile4 = tile_bytes / pl / 16;  // float4 per tile per plane
                 const float4 *src = reinterpret_cast<const float4 *>(A.Xlp) + (size_t)b * pl * MT_all * tile4;
-                for (int e = threadIdx.x; e < n_f4; e += 256) {
+                for (int e = tid; e < n_f4; e += 256) {
                     const int p = e / (MT * tile4), rem = e - p * (MT * tile4);
                     const int mt = rem / tile4;
                     dst[e] = (mt < mt_here) ? src[((size_t)p * MT_all + mt0 + mt) * tile4 + (rem - mt * tile4)]
@@ -811,7 +826,7 @@ void beamform_kernel(BeamArgs A)
                     atomicAdd(&A.image[(size_t)f_run * n_vox + v].y, A.out_scale * si);
                 }
             }
-            if (g == 0 && b_end == A.n_f && blockIdx.z == 0 && A.weights) A.weights[v] = w_last;
+            if (g == 0 && b_end == A.n_f && bz == 0 && A.weights) A.weights[v] = w_last;
         }
         return;
     }
@@ -839,12 +854,51 @@ void beamform_kernel(BeamArgs A)
             A.image[v] = val;
             if (TAN) dimg[v] = make_float2(A.out_scale * Dr, A.out_scale * Di);
         }
-        if (b_end == A.n_f && blockIdx.z == 0) {
+        if (b_end == A.n_f && bz == 0) {
             if (A.weights) A.weights[v] = w_last;
             if (MODE == MODE_SA && A.weights_tx) A.weights_tx[v] = wtx_last;
         }
     }
 }
+
+template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<MT, KI>::value)))
+void beamform_kernel(BeamArgs A)
+{
+    beamform_tile<MODE, MT, KI, WCR, WCT, PREC, TAN, ENS, NW>(A, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
+}
+
+// Persistent batch form: the work items (job, voxel tile, bin chunk, transmit
+// group) of every plan of one kernel class -- uneven grids, angle counts and band
+// chunkings -- in one launch.  Workgroups claim items from a counter (reset to 0
+// before the launch) until the list is exhausted, so every workgroup exits.
+template <int MODE, int MT, int KI, int WCR, int WCT, int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PersistWavesPerEU<MODE, MT, KI, WCR, PREC>::value)))
+void beamform_persistent_kernel(const BeamArgs *__restrict__ jobs, const int4 *__restrict__ items, int n_items,
+                                int *__restrict__ counter)
+{
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        // wave-uniform item: its table row and the job's arguments are read with scalar loads
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();  // read by every thread before the next claim overwrites it
+        if (it >= n_items) return;
+        const int4 w = items[it];
+        const BeamArgs &A = jobs[__builtin_amdgcn_readfirstlane(w.x)];
+        // the thread index, opaque to the compiler per item: values derived from it are
+        // recomputed per tile rather than hoisted out of the loop and held across tiles
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        beamform_tile<MODE, MT, KI, WCR, WCT, PREC>(A, __builtin_amdgcn_readfirstlane(w.y),
+                                                    __builtin_amdgcn_readfirstlane(w.z),
+                                                    __builtin_amdgcn_readfirstlane(w.w), tid);
+    }
+}
+
+static __global__ void reset_counter_kernel(int *c) { *c = 0; }
+
 
 // -------------------------------------------------------------------------
 // 16-bit spectra planes for the mixed-precision contraction

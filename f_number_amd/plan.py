@@ -256,11 +256,17 @@ class Batch:
     replayed by one graph launch per execute(); a change of any buffer pointer
     captures again."""
 
-    def __init__(self, plans, n_streams: int = 4):
+    def __init__(self, plans, n_streams: int = 4, persistent: bool = True):
         self.plans = list(plans)
         arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
         self._h = C.c_void_p()
         _lib.check(_lib.load().fn_batch_create(arr, len(self.plans), int(n_streams), C.byref(self._h)), "fn_batch")
+        self.set_persistent(persistent)
+
+    def set_persistent(self, enable: bool = True):
+        """Plans of one kernel class as one persistent launch over all their work items
+        (fnumber_hip.h fn_batch_set_persistent); False: one launch per plan."""
+        _lib.check(_lib.load().fn_batch_set_persistent(self._h, int(bool(enable))), "fn_batch")
 
     def execute(self, rfs, images, stream=None):
         n = len(self.plans)

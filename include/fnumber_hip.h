@@ -262,6 +262,16 @@ typedef struct fn_batch fn_batch;
 int32_t fn_batch_create(fn_plan *const *plans, int32_t n_plans, int32_t n_streams, fn_batch **batch);
 int32_t fn_batch_execute(fn_batch *batch, const void *const *d_rf, const int32_t *rf_dtypes, float *const *d_images,
                          void *stream);
+/* Plans that share a kernel class (mode, element count class, transmit tiles,
+ * window classes, precision; not ensembles or windowed normalization) are
+ * beamformed by persistent launches over all their work items -- uneven grids,
+ * angle counts and bin chunkings in one kernel whose workgroups claim
+ * (plan, voxel tile, bin chunk, transmit group) items from a device counter,
+ * longest chunks first.  A class is cut into launches of at most 8 plans, each
+ * waiting only for its own plans' spectra.  enable = 0 launches every plan on
+ * its own (each still a graph node); the next execute captures again.
+ * Default: enabled. */
+int32_t fn_batch_set_persistent(fn_batch *batch, int32_t enable);
 /* graph nodes and the number of captures so far */
 int32_t fn_batch_info(const fn_batch *batch, int64_t *n_nodes, int32_t *n_captures);
 int32_t fn_batch_destroy(fn_batch *batch);

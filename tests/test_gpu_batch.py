@@ -192,3 +192,58 @@ def test_batch_graph_replays_match_eager(fa, torch):
     batch.close()
     for j in jobs:
         j[0].close()
+
+
+def test_persistent_batch_matches_eager(fa, torch):
+    """Plans sharing a kernel class (uneven grids, angle counts and band splits) run as
+    one persistent launch over their work items: every replay equals the eager images,
+    with persistent launches on and off, and the persistent graph has fewer nodes."""
+    from f_number_amd import synth
+
+    specs = [  # (n_x, n_z, angles, precision, (m_tiles, bin_split) or None = default tiling)
+        (48, 40, 11, "fp32", (1, 1)), (32, 64, 1, "fp32", (1, 4)), (64, 32, 3, "fp32", (1, 2)),
+        (40, 40, 11, "fp32", None), (130, 24, 5, "fp32", None),
+        (28, 36, 24, "fp32", (1, 3)),  # two transmit groups of one tile each
+        (36, 28, 31, "fp32", None), (44, 20, 17, "fp32", None),  # two tiles
+        (48, 40, 11, "fp16", None), (56, 24, 3, "fp16", (1, 2)), (24, 56, 1, "fp16", None)]
+    jobs = []
+    for k, (nx, nz, na, prec, tiling) in enumerate(specs):
+        w = _cpwc(nx, nz, na, 500 + k)
+        plan = _plan(fa, w, precision=prec)
+        if tiling is not None:
+            plan.set_tiling(*tiling)
+        jobs.append((plan, w.rf_device("cuda:0"), (nx, nz)))
+    for nx, nz in [(20, 16), (28, 12)]:
+        ws = synth.scaled(synth.saft_full(), nx, nz, n_samples=768, n_speckle=300, n_elements=32)
+        ws.pitch = 4 * 0.3048e-3
+        ws.width = 4 * 0.27e-3
+        sa = fa.SaPlan(ws.positions_x, ws.positions_z, ws.n_samples, ws.n_elements, ws.f_s, ws.width, ws.pitch,
+                       ws.c_0, ws.f_bounds, 0.0, output_scale="cpu")
+        jobs.append((sa, ws.rf_device("cuda:0"), (nx, nz)))
+    eager, outs = [], []
+    for plan, rf, shape in jobs:
+        e = torch.empty(shape, dtype=torch.complex64, device="cuda")
+        plan.execute(rf, e)
+        eager.append(e)
+        outs.append(torch.empty_like(e))
+    torch.cuda.synchronize()
+    rfs = [j[1] for j in jobs]
+    batch = fa.Batch([j[0] for j in jobs], n_streams=3, persistent=True)
+    nodes = {}
+    for persistent in (True, False):
+        batch.set_persistent(persistent)
+        print(f"case persistent={persistent}", flush=True)
+        for replay in range(3):
+            for o in outs:
+                o.fill_(float("nan"))  # eager work between replays
+            batch.execute(rfs, outs)
+            torch.cuda.synchronize()
+            for i, (o, e) in enumerate(zip(outs, eager)):
+                err = (torch.linalg.vector_norm(o - e) / torch.linalg.vector_norm(e)).item()
+                assert err < 1e-6, f"persistent={persistent} replay {replay} job {i}: {err}"
+        nodes[persistent] = batch.info()["graph_nodes"]
+    print(f"case graph nodes: {nodes}", flush=True)
+    assert nodes[True] < nodes[False]
+    batch.close()
+    for j in jobs:
+        j[0].close()

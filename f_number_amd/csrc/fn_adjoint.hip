@@ -39,6 +39,7 @@ namespace fnk {
 constexpr int ADJ_WAVES = 4;   // bins per block, one per wave (two blocks per CU)
 constexpr int ADJ_ET = 8;      // element tiles of 16 (128 elements)
 constexpr int ADJ_QS = 132;    // LDS words per voxel of the receive phase table ([cidx][mt] + pad)
+constexpr int ADJ_GV = 32;     // voxels per LDS group (one barrier each): two 16-voxel contraction groups
 
 // FULL: no per-element support test on a 128-element array (no padded
 // elements).  Either the support is every element (the test reduces to the
@@ -51,10 +52,11 @@ template <int WCR, bool FULL>
 __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(AdjArgs S)
 {
     // group tables, double-buffered: the next group is loaded while this one is contracted
-    __shared__ __align__(16) uint32_t q_rx[2][16 * ADJ_QS];  // [voxel i][cidx][mt]: element cidx + 16 mt
-    __shared__ uint32_t q_tx[2][16 * 16];      // [voxel i][angle column]
-    __shared__ float2 vy[2][16];               // cotangent of the group's voxels (0 outside the range)
+    __shared__ __align__(16) uint32_t q_rx[2][ADJ_GV * ADJ_QS];  // [voxel i][cidx][mt]: element cidx + 16 mt
+    __shared__ uint32_t q_tx[2][ADJ_GV * 16];      // [voxel i][angle column]
+    __shared__ float2 vy[2][ADJ_GV];               // cotangent of the group's voxels (0 outside the range)
     // per wave, for 64 voxels: (klo | khi << 16, 1/W_L, 1/W_R, x_0 - x) with x_0 = -M pitch
+    // (for WC_COS_RF the two inverse half-widths carry the window slope q_s)
     __shared__ __align__(16) uint4 apx[ADJ_WAVES][64];
     const BeamArgs &A = S.B;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -82,26 +84,47 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     }
 
     const int qtx_stride = 16 * S.n_at;
-    // copy the phase words and voxel data of the group at `base` into buffer `buf`
-    static_assert(ADJ_WAVES == 4, "group loader assumes 256 threads");
-    auto load_group = [&](long base, int buf) {
+    // the phase words and voxel data of the group at `base`: fetched into registers
+    // at the top of the previous group's step and written to LDS buffer `buf` after
+    // its contraction, so the global loads are in flight during that contraction
+    static_assert(ADJ_WAVES == 4 && ADJ_GV % 16 == 0, "group loader assumes 256 threads");
+    constexpr int RX4 = ADJ_GV * 32 / 256, TX1 = ADJ_GV * 16 / 256;  // uint4 / words per thread
+    struct GroupRegs {
+        uint4 rx[RX4];
+        uint32_t tx[TX1];
+        float2 y;
+    };
+    auto fetch_group = [&](long base) {
+        GroupRegs r;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int it = tid + 256 * h, i = it >> 5, j4 = it & 31;  // 16 voxels x 32 uint4
+        for (int h = 0; h < RX4; ++h) {
+            const int it = tid + 256 * h, i = it >> 5, j4 = it & 31;  // ADJ_GV voxels x 32 uint4
             const long u = base + i;
-            uint4 w = make_uint4(0u, 0u, 0u, 0u);
-            if (u < n_vox) w = reinterpret_cast<const uint4 *>(S.qrx + (size_t)u * 128)[j4];
-            *reinterpret_cast<uint4 *>(&q_rx[buf][i * ADJ_QS + 4 * j4]) = w;
+            r.rx[h] = make_uint4(0u, 0u, 0u, 0u);
+            if (u < n_vox) r.rx[h] = reinterpret_cast<const uint4 *>(S.qrx + (size_t)u * 128)[j4];
         }
-        {
-            const int iv = tid >> 4, col = tid & 15;  // 16 voxels x 16 angle columns
+#pragma unroll
+        for (int h = 0; h < TX1; ++h) {
+            const int it = tid + 256 * h, iv = it >> 4, col = it & 15;  // ADJ_GV voxels x 16 angle columns
             const long uv = base + iv;
-            q_tx[buf][tid] = uv < n_vox ? S.qtx[(size_t)uv * qtx_stride + 16 * at + col] : 0u;
+            r.tx[h] = uv < n_vox ? S.qtx[(size_t)uv * qtx_stride + 16 * at + col] : 0u;
         }
-        if (tid < 16) {
+        r.y = make_float2(0.f, 0.f);
+        if (tid < ADJ_GV) {
             const long u2 = base + tid;
-            vy[buf][tid] = u2 < vr1 ? S.y[u2] : make_float2(0.f, 0.f);
+            if (u2 < vr1) r.y = S.y[u2];
         }
+        return r;
+    };
+    auto store_group = [&](const GroupRegs &r, int buf) {
+#pragma unroll
+        for (int h = 0; h < RX4; ++h) {
+            const int it = tid + 256 * h, i = it >> 5, j4 = it & 31;
+            *reinterpret_cast<uint4 *>(&q_rx[buf][i * ADJ_QS + 4 * j4]) = r.rx[h];
+        }
+#pragma unroll
+        for (int h = 0; h < TX1; ++h) q_tx[buf][tid + 256 * h] = r.tx[h];
+        if (tid < ADJ_GV) vy[buf][tid] = r.y;
     };
     // per-voxel aperture terms of voxel `base + lane` (clamped into the grid)
     auto load_geo = [&](long base) {
@@ -112,22 +135,31 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     const double x0 = -A.M * A.pitch;
     double4 geo = make_double4(0.0, 0.0, 0.0, 0.0);
     if (vr0 < vr1) {
-        load_group(vr0, 0);
+        store_group(fetch_group(vr0), 0);
         if (active) geo = load_geo(vr0);
     }
     __syncthreads();
     int cur = 0;
-    for (long base = vr0; base < vr1; base += 16, cur ^= 1) {
-        if (base + 16 < vr1) load_group(base + 16, cur ^ 1);  // the buffer last read two groups ago
+    for (long gbase = vr0; gbase < vr1; gbase += ADJ_GV, cur ^= 1) {
+        const bool more = gbase + ADJ_GV < vr1;
+        GroupRegs next{};
+        if (more) next = fetch_group(gbase + ADJ_GV);
+#pragma unroll
+        for (int hb = 0; hb < ADJ_GV / 16; ++hb) {
+        const long base = gbase + 16 * hb;  // contraction group of 16 voxels
+        const int o = 16 * hb;              // its offset in the LDS group
         const int gq = (int)((base - vr0) >> 4) & 3;           // group within the 64 voxels of apx
-        if (active) {
+        if (active && base < vr1) {
             if (gq == 0) {
                 // apertures of this wave's bin for voxels base .. base + 63, one per lane (forward formulas)
                 double Fr = Fb;
                 if (A.clamp) Fr = fmin(Fr, sqf * geo.w);  // das_pw.m:374-377
                 const Aperture ap = make_aperture(geo.x, geo.y, geo.z, Fr, A, A.support_rx);
-                apx[wave][lane] = make_uint4((uint32_t)ap.klo | ((uint32_t)ap.khi << 16), __float_as_uint(ap.inv_wl),
-                                             __float_as_uint(ap.inv_wr), __float_as_uint((float)(x0 - geo.x)));
+                // WC_COS_RF keeps the quarter-period slope folded in: q_s / W per side
+                const float sl = WCR == WC_COS_RF ? A.wrx.q_s : 1.0f;
+                apx[wave][lane] = make_uint4((uint32_t)ap.klo | ((uint32_t)ap.khi << 16),
+                                             __float_as_uint(ap.inv_wl * sl), __float_as_uint(ap.inv_wr * sl),
+                                             __float_as_uint((float)(x0 - geo.x)));
                 if (base + 64 < vr1) geo = load_geo(base + 64);  // consumed four groups later
             }
 
@@ -161,8 +193,13 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
                     }
                     const int e = cidx + 16 * mt;
                     const float d = fmaf((float)(16 * mt), pf, dg);
-                    const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
-                    float w = window_eval<WCR>(t, A.wrx);
+                    float w;
+                    if (WCR == WC_COS_RF) {
+                        w = window_rf(fabsf(d), d < 0.f ? iwl : iwr, A.wrx.q_o);
+                    } else {
+                        const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
+                        w = window_eval<WCR>(t, A.wrx);
+                    }
                     if (!FULL) w = (e >= klo && e <= khi) ? w : 0.f;
                     wv[mt] = w;
                     asum += w;
@@ -172,14 +209,14 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
                 const float sc =
                     (klo <= khi && a_ok) ? A.out_scale * (A.norm_rx ? __builtin_amdgcn_rcpf(asum) : 1.0f) : 0.f;
                 // U = s y conj(T) / w
-                const float2 yv = vy[cur][i];
-                const float rt = rev_of(n * q_tx[cur][i * 16 + cidx]);
+                const float2 yv = vy[cur][o + i];
+                const float rt = rev_of(n * q_tx[cur][(o + i) * 16 + cidx]);
                 const float st = __builtin_amdgcn_sinf(rt), ct = __builtin_amdgcn_cosf(rt);
                 const float ur = sc * (yv.x * ct + yv.y * st);
                 const float ui = sc * (yv.y * ct - yv.x * st);
                 const float u2 = ui - ur, u3 = ur + ui;
-                const uint4 qa = *reinterpret_cast<const uint4 *>(&q_rx[cur][i * ADJ_QS + cidx * ADJ_ET]);
-                const uint4 qb = *reinterpret_cast<const uint4 *>(&q_rx[cur][i * ADJ_QS + cidx * ADJ_ET + 4]);
+                const uint4 qa = *reinterpret_cast<const uint4 *>(&q_rx[cur][(o + i) * ADJ_QS + cidx * ADJ_ET]);
+                const uint4 qb = *reinterpret_cast<const uint4 *>(&q_rx[cur][(o + i) * ADJ_QS + cidx * ADJ_ET + 4]);
                 const uint32_t Q[ADJ_ET] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
     #pragma unroll
                 for (int mt = 0; mt < ADJ_ET; ++mt) {
@@ -193,7 +230,10 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
                 }
             }
         }  // active
-        __syncthreads();  // group `base` consumed, group `base + 16` loaded
+        }  // hb
+        // buffer cur ^ 1 was last read two groups ago (before the previous barrier)
+        if (more) store_group(next, cur ^ 1);
+        __syncthreads();  // group `gbase` consumed, group `gbase + ADJ_GV` stored
     }
     if (!active) return;
     // lane holds D[4q + r][cidx] = Z(element 16 mt + 4q + r, angle column cidx) of the range;
@@ -269,12 +309,15 @@ __global__ __launch_bounds__(256) void adjoint_reduce_kernel(const float *__rest
     Z[((size_t)b * n_tx + a) * n_el + k] = make_float2(re, im);
 }
 
-// (A^H y)[ch][t] = Re sum_b Z[b][ch] e^{+j 2 pi (n_lb + b) t / N}, ch = a n_el + k.
-// blockIdx.y = group of IDFT_CH consecutive channels (the Z reads of a bin are
-// uniform across the block), blockIdx.x * 256 + tid = sample; twiddles
-// e^{+j 2 pi m / N} in LDS, index (n t) mod N advanced exactly in integers and
-// shared by the channels of a thread.
-constexpr int IDFT_CH = 8;
+// (A^H y)[ch][t] = Re sum_b Z[b][ch] e^{+j 2 pi (n_lb + b) t / N}, ch = a n_el + k, as a
+// real product on the matrix cores: D[t][ch] += sum_k A[t][k] B[k][ch] over k = (bin, part),
+//   A[t][(b, 0)] = Re w,  A[t][(b, 1)] = -Im w,  w = e^{+j 2 pi m / N}, m = (n_lb + b) t mod N,
+//   B[(b, 0)][ch] = Re Z[b][ch],  B[(b, 1)][ch] = Im Z[b][ch].
+// v_mfma_f32_16x16x4f32: lane l supplies A[t0 + l % 16][k0 + l / 16] and
+// B[k0 + l / 16][c0 + l % 16], so one K step of 4 covers two bins.  A block of 4 waves
+// owns 128 samples x 64 channels; a wave 32 samples (2 tiles) x 64 channels (4 tiles).
+// Twiddles in LDS; each lane advances its index m by 2t mod N per step in integers.
+constexpr int IDFT_WT = 2, IDFT_WC = 4;
 
 __global__ __launch_bounds__(256) void adjoint_idft_kernel(const float2 *__restrict__ Z, const float2 *__restrict__ tw,
                                                            float *__restrict__ out, int n_t, int n_ch, int N, int n_lb,
@@ -283,29 +326,66 @@ __global__ __launch_bounds__(256) void adjoint_idft_kernel(const float2 *__restr
     extern __shared__ float2 tws[];
     for (int m = threadIdx.x; m < N; m += 256) tws[m] = tw[m];
     __syncthreads();
-    const int ch0 = blockIdx.y * IDFT_CH;
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n_t) return;
-    const int n_here = min(IDFT_CH, n_ch - ch0);
-    long m = ((long)n_lb * t) % N;
-    float acc[IDFT_CH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, q = lane >> 4;  // q = k within the step: bin q / 2, part q % 2
+    const int part = q & 1, db = q >> 1;
+    const int t0 = blockIdx.x * 128 + wave * 32;
+    const int c0 = blockIdx.y * 64;
+    int m[IDFT_WT], dm[IDFT_WT];
+    bool tok[IDFT_WT];
 #pragma unroll
-    for (int c = 0; c < IDFT_CH; ++c) acc[c] = 0.f;
-    for (int b = 0; b < n_f; ++b) {
-        const float2 w = tws[m];
-        const float2 *zb = Z + (size_t)b * n_ch + ch0;
-#pragma unroll
-        for (int c = 0; c < IDFT_CH; ++c) {
-            const float2 z = c < n_here ? zb[c] : make_float2(0.f, 0.f);
-            acc[c] = fmaf(z.x, w.x, acc[c]);
-            acc[c] = fmaf(-z.y, w.y, acc[c]);
-        }
-        m += t;
-        if (m >= N) m -= N;
+    for (int i = 0; i < IDFT_WT; ++i) {
+        const int t = t0 + 16 * i + r16;
+        tok[i] = t < n_t;
+        m[i] = (int)(((long)(n_lb + db) * t) % N);
+        dm[i] = (int)((2L * t) % N);
     }
+    int ch[IDFT_WC];
 #pragma unroll
-    for (int c = 0; c < IDFT_CH; ++c)
-        if (c < n_here) out[(size_t)(ch0 + c) * n_t + t] = acc[c];
+    for (int j = 0; j < IDFT_WC; ++j) ch[j] = c0 + 16 * j + r16;
+    floatx4 acc[IDFT_WT][IDFT_WC];
+#pragma unroll
+    for (int i = 0; i < IDFT_WT; ++i)
+#pragma unroll
+        for (int j = 0; j < IDFT_WC; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int b0 = 0; b0 < n_f; b0 += 2) {
+        const int b = b0 + db;
+        float a[IDFT_WT], bv[IDFT_WC];
+#pragma unroll
+        for (int i = 0; i < IDFT_WT; ++i) {
+            const float2 w = tws[m[i]];
+            a[i] = tok[i] ? (part ? -w.y : w.x) : 0.f;
+            m[i] += dm[i];
+            if (m[i] >= N) m[i] -= N;
+        }
+#pragma unroll
+        for (int j = 0; j < IDFT_WC; ++j) {
+            float2 z = make_float2(0.f, 0.f);
+            if (b < n_f && ch[j] < n_ch) z = Z[(size_t)b * n_ch + ch[j]];
+            bv[j] = part ? z.y : z.x;
+        }
+#pragma unroll
+        for (int i = 0; i < IDFT_WT; ++i)
+#pragma unroll
+            for (int j = 0; j < IDFT_WC; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    // lane holds D[t0 + 16 i + 4 q + r][ch[j]], r = 0..3: four consecutive samples of one channel
+    const bool vec = (n_t & 3) == 0;
+#pragma unroll
+    for (int i = 0; i < IDFT_WT; ++i)
+#pragma unroll
+        for (int j = 0; j < IDFT_WC; ++j) {
+            const int t = t0 + 16 * i + 4 * q;
+            if (ch[j] >= n_ch || t >= n_t) continue;
+            float *dst = out + (size_t)ch[j] * n_t + t;
+            if (vec) {  // t + 3 < n_t and 16-byte aligned
+                *reinterpret_cast<float4 *>(dst) = make_float4(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (t + r < n_t) dst[r] = acc[i][j][r];
+            }
+        }
 }
 
 // d J / d delay_k = sum_b w_b sum_a (Xr Zi - Xi Zr); one block per element, FP64 sums.
@@ -419,7 +499,7 @@ hipError_t launch_adjoint_idft(const float2 *Z, const float2 *tw, float *out, in
         if (e != hipSuccess) return e;
         lds_done = lds;
     }
-    hipLaunchKernelGGL(adjoint_idft_kernel, dim3((unsigned)((n_t + 255) / 256), (unsigned)((n_ch + IDFT_CH - 1) / IDFT_CH)),
+    hipLaunchKernelGGL(adjoint_idft_kernel, dim3((unsigned)((n_t + 127) / 128), (unsigned)((n_ch + 63) / 64)),
                        dim3(256), lds, s, Z, tw, out, n_t, n_ch, N, n_lb, n_f);
     return hipGetLastError();
 }

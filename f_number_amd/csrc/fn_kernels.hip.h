@@ -213,6 +213,15 @@ inline bool range_free(const BeamArgs &A, int n_full)
     return A.support_rx == SUPPORT_SNAPPED && A.wrx.t_cut <= 1.0f && 0.5 * A.width <= 0.999 * A.pitch;
 }
 
+// WC_COS_RF with the quarter-period slope folded into the inverse half-width:
+// ad = |x_k - x|, iw_q = q_s / W on the element's side; the same value as
+// window_eval<WC_COS_RF>(ad / W) with one multiplication less per element.
+__device__ __forceinline__ float window_rf(float ad, float iw_q, float q_o)
+{
+    const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(fmaf(ad, iw_q, q_o), 0.f, 0.25f));
+    return c * c;
+}
+
 template <int WC>
 __device__ __forceinline__ float window_eval(float t, const WinCoef &w)
 {
@@ -426,6 +435,15 @@ struct WavesPerEU {
     static constexpr int value = (MT <= 2) ? 1 : (bytes <= 81920 ? 2 : 1);
 };
 
+// One-launch-per-plan kernel: the smallest tiles (MT x KI <= 32) are held to 3
+// waves per SIMD where they fit without spilling (not the ensemble epilogue, not
+// the 16-bit forms with the LINEAR or general COS windows); otherwise WavesPerEU.
+template <int MT, int KI, int WCR, int PREC, bool ENS>
+struct BeamWavesPerEU {
+    static constexpr bool fits3 = !ENS && (PREC == PREC_FP32 || (WCR != WC_LINEAR && WCR != WC_COS));
+    static constexpr int value = (MT * KI <= 32 && fits3) ? 3 : WavesPerEU<MT, KI>::value;
+};
+
 // Occupancy target of the persistent batch kernel.  The claim loop around the
 // tile costs registers; left alone the compiler settles small plane-wave tiles
 // (MT x KI <= 64, where LDS admits 3+ workgroups per CU) at 2 waves per SIMD
@@ -435,7 +453,7 @@ struct WavesPerEU {
 template <int MODE, int MT, int KI, int WCR, int PREC>
 struct PersistWavesPerEU {
     static constexpr bool fits3 = (PREC == PREC_FP32) ? (WCR == WC_COS_ALL) : (WCR != WC_LINEAR && WCR != WC_COS);
-    static constexpr int value = (MODE == MODE_PW && MT * KI <= 64 && fits3) ? 3 : WavesPerEU<MT, KI>::value;
+    static constexpr int value = (MODE == MODE_PW && MT <= 3 && MT * KI <= 64 && fits3) ? 3 : WavesPerEU<MT, KI>::value;
 };
 
 // TAN (plane waves, FP32): also accumulates the tangent dI/dc_tof.  Geometric
@@ -449,9 +467,9 @@ struct PersistWavesPerEU {
 // One workgroup's share of a plan: voxel tile bx (64 voxels), bin chunk by,
 // transmit group bz.  beamform_kernel runs it for its block index; the
 // persistent batch kernel for every work item it claims.
-template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false>
-__device__ __forceinline__ void beamform_tile(const BeamArgs &A, const int bx, const int by, const int bz,
-                                              const int tid)
+template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false,
+          typename Args = BeamArgs>
+__device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by, const int bz, const int tid)
 {
     static_assert(!TAN || (MODE == MODE_PW && PREC == PREC_FP32), "tangent: plane waves, FP32");
     static_assert(!ENS || (MODE == MODE_PW && !TAN), "ensemble: plane waves");
@@ -598,6 +616,7 @@ __device__ __forceinline__ void beamform_tile(const BeamArgs &A, const int bx, c
             const int klo = ap.klo - g, khi = ap.khi - g;  // support in units of 4j
             const float dstep = 4.0f * pf;
             const float ul = ap.inv_wl * A.wrx.s_half, ur = ap.inv_wr * A.wrx.s_half;
+            const float ql = ap.inv_wl * A.wrx.q_s, qr = ap.inv_wr * A.wrx.q_s;  // WC_COS_RF slopes
             const int j_hi = min(u_hi >> 2, KI - 1);  // elements 4j + g, g = 0..3
             for (int j = u_lo >> 2; j <= j_hi; ++j) {
                 const float d = fmaf((float)j, dstep, dgb);
@@ -606,6 +625,8 @@ __device__ __forceinline__ void beamform_tile(const BeamArgs &A, const int bx, c
                     float u = fabsf(d) * (d < 0.f ? ul : ur);  // revolutions
                     u = u - rintf(u);
                     a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
+                } else if (WCR == WC_COS_RF) {
+                    a = window_rf(fabsf(d), d < 0.f ? ql : qr, A.wrx.q_o);
                 } else {
                     const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
                     a = window_eval<WCR>(t, A.wrx);
@@ -656,6 +677,8 @@ __device__ __forceinline__ void beamform_tile(const BeamArgs &A, const int bx, c
                 accR[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
                 accI[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
+            const float ul = ap.inv_wl * A.wrx.s_half, ur = ap.inv_wr * A.wrx.s_half;
+            const float ql = ap.inv_wl * A.wrx.q_s, qr = ap.inv_wr * A.wrx.q_s;  // WC_COS_RF slopes
 #pragma unroll
             for (int cc = 0; cc < KC; ++cc) {
                 if (cc < (u_lo >> 5) || cc > (u_hi >> 5)) continue;  // elements 32 cc .. 32 cc + 31
@@ -667,9 +690,11 @@ __device__ __forceinline__ void beamform_tile(const BeamArgs &A, const int bx, c
                     const float d = fmaf((float)(k - k0), pf, dgb);
                     float a;
                     if (WCR == WC_COS_ALL) {
-                        float u = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr) * A.wrx.s_half;  // revolutions
+                        float u = fabsf(d) * (d < 0.f ? ul : ur);  // revolutions
                         u = u - rintf(u);
                         a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
+                    } else if (WCR == WC_COS_RF) {
+                        a = window_rf(fabsf(d), d < 0.f ? ql : qr, A.wrx.q_o);
                     } else {
                         const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
                         a = window_eval<WCR>(t, A.wrx);
@@ -862,10 +887,15 @@ __device__ __forceinline__ void beamform_tile(const BeamArgs &A, const int bx, c
 }
 
 template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerEU<MT, KI>::value)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BeamWavesPerEU<MT, KI, WCR, PREC, ENS>::value)))
 void beamform_kernel(BeamArgs A)
 {
-    beamform_tile<MODE, MT, KI, WCR, WCT, PREC, TAN, ENS, NW>(A, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
+    // small tiles take the arguments by value (their register count decides the
+    // occupancy); LDS holds large FP32 ones at <= 2 waves per SIMD, where the
+    // reference form leaves the compiler the registers it schedules best with
+    using Args =
+        typename std::conditional<(MT * KI <= 64 || PREC != PREC_FP32), BeamArgs, const BeamArgs &>::type;
+    beamform_tile<MODE, MT, KI, WCR, WCT, PREC, TAN, ENS, NW, Args>(A, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
 }
 
 // Persistent batch form: the work items (job, voxel tile, bin chunk, transmit
@@ -891,7 +921,8 @@ void beamform_persistent_kernel(const BeamArgs *__restrict__ jobs, const int4 *_
         // recomputed per tile rather than hoisted out of the loop and held across tiles
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        beamform_tile<MODE, MT, KI, WCR, WCT, PREC>(A, __builtin_amdgcn_readfirstlane(w.y),
+        beamform_tile<MODE, MT, KI, WCR, WCT, PREC, false, false, false, const BeamArgs &>(
+            A, __builtin_amdgcn_readfirstlane(w.y),
                                                     __builtin_amdgcn_readfirstlane(w.z),
                                                     __builtin_amdgcn_readfirstlane(w.w), tid);
     }

@@ -55,6 +55,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     __shared__ __align__(16) uint32_t q_rx[2][ADJ_GV * ADJ_QS];  // [voxel i][cidx][mt]: element cidx + 16 mt
     __shared__ uint32_t q_tx[2][ADJ_GV * 16];      // [voxel i][angle column]
     __shared__ float2 vy[2][ADJ_GV];               // cotangent of the group's voxels (0 outside the range)
+    __shared__ float scv[2][ADJ_WAVES][ADJ_GV];    // s / w_n(v) of each wave's bin for the group's voxels
     // per wave, for 64 voxels: (klo | khi << 16, 1/W_L, 1/W_R, x_0 - x) with x_0 = -M pitch
     // (for WC_COS_RF the two inverse half-widths carry the window slope q_s)
     __shared__ __align__(16) uint4 apx[ADJ_WAVES][64];
@@ -93,7 +94,9 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
         uint4 rx[RX4];
         uint32_t tx[TX1];
         float2 y;
+        float sc;
     };
+    static_assert(ADJ_WAVES * ADJ_GV <= 256, "one scale per thread");
     auto fetch_group = [&](long base) {
         GroupRegs r;
 #pragma unroll
@@ -114,6 +117,12 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
             const long u2 = base + tid;
             if (u2 < vr1) r.y = S.y[u2];
         }
+        r.sc = 0.f;
+        if (tid < ADJ_WAVES * ADJ_GV) {
+            const int bs = blockIdx.y * ADJ_WAVES + tid / ADJ_GV;
+            const long u3 = base + tid % ADJ_GV;
+            if (bs < A.n_f && u3 < n_vox) r.sc = S.vsc[(size_t)bs * n_vox + u3];
+        }
         return r;
     };
     auto store_group = [&](const GroupRegs &r, int buf) {
@@ -125,6 +134,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int h = 0; h < TX1; ++h) q_tx[buf][tid + 256 * h] = r.tx[h];
         if (tid < ADJ_GV) vy[buf][tid] = r.y;
+        if (tid < ADJ_WAVES * ADJ_GV) scv[buf][tid / ADJ_GV][tid % ADJ_GV] = r.sc;
     };
     // per-voxel aperture terms of voxel `base + lane` (clamped into the grid)
     auto load_geo = [&](long base) {
@@ -183,31 +193,8 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
                 const int klo = (int)(apv.x & 0xffffu), khi = (int)(apv.x >> 16);
                 const float iwl = __uint_as_float(apv.y), iwr = __uint_as_float(apv.z);
                 const float dg = fmaf((float)cidx, pf, __uint_as_float(apv.w));  // x_cidx - x
-                float wv[ADJ_ET];
-                float asum = 0.f;
-    #pragma unroll
-                for (int mt = 0; mt < ADJ_ET; ++mt) {
-                    if (mt < t_lo || mt > t_hi) {
-                        wv[mt] = 0.f;
-                        continue;
-                    }
-                    const int e = cidx + 16 * mt;
-                    const float d = fmaf((float)(16 * mt), pf, dg);
-                    float w;
-                    if (WCR == WC_COS_RF) {
-                        w = window_rf(fabsf(d), d < 0.f ? iwl : iwr, A.wrx.q_o);
-                    } else {
-                        const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
-                        w = window_eval<WCR>(t, A.wrx);
-                    }
-                    if (!FULL) w = (e >= klo && e <= khi) ? w : 0.f;
-                    wv[mt] = w;
-                    asum += w;
-                }
-                // weight sum over the 128 elements of voxel i: 16 lanes of the row (DPP)
-                asum = row_sum16(asum);
-                const float sc =
-                    (klo <= khi && a_ok) ? A.out_scale * (A.norm_rx ? __builtin_amdgcn_rcpf(asum) : 1.0f) : 0.f;
+                // s / w_n(v) of voxel i (0 outside a valid aperture), per-plan table
+                const float sc = a_ok ? scv[cur][wave][o + i] : 0.f;
                 // U = s y conj(T) / w
                 const float2 yv = vy[cur][o + i];
                 const float rt = rev_of(n * q_tx[cur][(o + i) * 16 + cidx]);
@@ -221,9 +208,19 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     #pragma unroll
                 for (int mt = 0; mt < ADJ_ET; ++mt) {
                     if (mt < t_lo || mt > t_hi) continue;
+                    const int e = cidx + 16 * mt;
+                    const float d = fmaf((float)(16 * mt), pf, dg);
+                    float w;
+                    if (WCR == WC_COS_RF) {
+                        w = window_rf(fabsf(d), d < 0.f ? iwl : iwr, A.wrx.q_o);
+                    } else {
+                        const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
+                        w = window_eval<WCR>(t, A.wrx);
+                    }
+                    if (!FULL) w = (e >= klo && e <= khi) ? w : 0.f;
                     const float rv = rev_of(n * Q[mt]);
                     const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                    const float br = wv[mt] * cs, bi = wv[mt] * sn;
+                    const float br = w * cs, bi = w * sn;
                     K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br - bi, ur, K1[mt], 0, 0, 0);
                     K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, u2, K2[mt], 0, 0, 0);
                     K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi, u3, K3[mt], 0, 0, 0);
@@ -445,7 +442,60 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restr
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-hipError_t launch_adjoint(int wcr, bool tables, dim3 grid, hipStream_t s, const AdjArgs &S)
+// Normalization of the voxel contraction per (bin, voxel): s / w_n(v) for a valid
+// receive aperture, 0 otherwise, with w_n(v) the sum of the apodization weights over
+// the array (das_pw.m:400-410, normalizations.on; s alone for off).  It depends on
+// the geometry, the F-numbers and the window only -- not on the time of flight --
+// and is computed once per plan, with the contraction kernel's aperture and element
+// arithmetic (elements cidx + 16 mt, FULL as there).
+template <int WCR, bool FULL>
+__global__ __launch_bounds__(256) void adjoint_scale_kernel(BeamArgs A, float *__restrict__ vsc)
+{
+    const long n_vox = (long)A.n_x * A.n_z;
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (v >= n_vox) return;
+    const int ix = (int)(v / A.n_z), iz = (int)(v - (long)ix * A.n_z);
+    const double x = A.px[ix], z = A.pz[iz];
+    double Fr = A.Frx[b];
+    if (A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * sqrt(z / (2.88 * A.c0)));  // das_pw.m:374-377
+    const Aperture ap = make_aperture(x, x / A.pitch, z / (2.0 * A.pitch), Fr, A, A.support_rx);
+    float sc = 0.f;
+    if (ap.klo <= ap.khi) {
+        const float pf = (float)A.pitch;
+        const float x0f = (float)(-A.M * A.pitch - x);
+        const float sl = WCR == WC_COS_RF ? A.wrx.q_s : 1.0f;
+        const float iwl = ap.inv_wl * sl, iwr = ap.inv_wr * sl;
+        float asum = 0.f;
+        for (int e = 0; e < 16 * ADJ_ET; ++e) {
+            const int cidx = e & 15, mt = e >> 4;
+            const float d = fmaf((float)(16 * mt), pf, fmaf((float)cidx, pf, x0f));
+            float w;
+            if (WCR == WC_COS_RF) {
+                w = window_rf(fabsf(d), d < 0.f ? iwl : iwr, A.wrx.q_o);
+            } else {
+                w = window_eval<WCR>(fabsf(d) * (d < 0.f ? iwl : iwr), A.wrx);
+            }
+            if (!FULL) w = (e >= ap.klo && e <= ap.khi) ? w : 0.f;
+            asum += w;
+        }
+        sc = A.out_scale * (A.norm_rx ? 1.0f / asum : 1.0f);
+    }
+    vsc[(size_t)b * n_vox + v] = sc;
+}
+
+template <int WCR, bool FULL>
+static void launch_adjoint_class(bool scale, dim3 grid, hipStream_t s, const AdjArgs &S)
+{
+    if (scale) {
+        const long n_vox = (long)S.B.n_x * S.B.n_z;
+        hipLaunchKernelGGL((adjoint_scale_kernel<WCR, FULL>), dim3((unsigned)((n_vox + 255) / 256), (unsigned)S.B.n_f),
+                           dim3(256), 0, s, S.B, const_cast<float *>(S.vsc));
+    }
+    hipLaunchKernelGGL((adjoint_kernel<WCR, FULL>), grid, dim3(64 * ADJ_WAVES), 0, s, S);
+}
+
+hipError_t launch_adjoint(int wcr, bool tables, bool scale, dim3 grid, hipStream_t s, const AdjArgs &S)
 {
     const long n_vox = (long)S.B.n_x * S.B.n_z;
     const long n_tab = n_vox * (128 + 16 * S.n_at + 1);
@@ -453,27 +503,26 @@ hipError_t launch_adjoint(int wcr, bool tables, dim3 grid, hipStream_t s, const 
         hipLaunchKernelGGL(adjoint_tables_kernel, dim3((unsigned)((n_tab + 255) / 256)), dim3(256), 0, s, S.B,
                            const_cast<uint32_t *>(S.qrx), const_cast<uint32_t *>(S.qtx), const_cast<double *>(S.vgeo),
                            S.n_at);
-    const dim3 block(64 * ADJ_WAVES);
     const bool full = range_free(S.B, 16 * ADJ_ET);
     // the forward's element-loop classes: WC_COS_RF holds when the support test is
     // free here too (128 elements); WC_COS_ALL evaluates as the general cosine
     if (wcr == WC_COS_RF && full) {
-        hipLaunchKernelGGL((adjoint_kernel<WC_COS_RF, true>), grid, block, 0, s, S);
+        launch_adjoint_class<WC_COS_RF, true>(scale, grid, s, S);
         return hipGetLastError();
     }
     if (wcr == WC_COS_RF || wcr == WC_COS_ALL) wcr = WC_COS;
     switch (wcr) {
     case WC_RANGE:
-        if (full) hipLaunchKernelGGL((adjoint_kernel<WC_RANGE, true>), grid, block, 0, s, S);
-        else hipLaunchKernelGGL((adjoint_kernel<WC_RANGE, false>), grid, block, 0, s, S);
+        if (full) launch_adjoint_class<WC_RANGE, true>(scale, grid, s, S);
+        else launch_adjoint_class<WC_RANGE, false>(scale, grid, s, S);
         break;
     case WC_LINEAR:
-        if (full) hipLaunchKernelGGL((adjoint_kernel<WC_LINEAR, true>), grid, block, 0, s, S);
-        else hipLaunchKernelGGL((adjoint_kernel<WC_LINEAR, false>), grid, block, 0, s, S);
+        if (full) launch_adjoint_class<WC_LINEAR, true>(scale, grid, s, S);
+        else launch_adjoint_class<WC_LINEAR, false>(scale, grid, s, S);
         break;
     default:
-        if (full) hipLaunchKernelGGL((adjoint_kernel<WC_COS, true>), grid, block, 0, s, S);
-        else hipLaunchKernelGGL((adjoint_kernel<WC_COS, false>), grid, block, 0, s, S);
+        if (full) launch_adjoint_class<WC_COS, true>(scale, grid, s, S);
+        else launch_adjoint_class<WC_COS, false>(scale, grid, s, S);
         break;
     }
     return hipGetLastError();

@@ -75,7 +75,7 @@ hipError_t launch_focused_signal(const BeamArgs &A, int mode, int wcr, int wct, 
 hipError_t launch_beamform_tan_k8(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k16(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k32(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
-hipError_t launch_adjoint(int wcr, bool tables, dim3 grid, hipStream_t s, const AdjArgs &S);
+hipError_t launch_adjoint(int wcr, bool tables, bool scale, dim3 grid, hipStream_t s, const AdjArgs &S);
 hipError_t launch_adjoint_reduce(const float *P, float2 *Z, int n_vr, int n_f, int n_at, int n_tx, int n_el,
                                  hipStream_t s);
 hipError_t launch_adjoint_idft(const float2 *Z, const float2 *tw, float *out, int n_t, int n_ch, int N, int n_lb,
@@ -149,6 +149,9 @@ struct fn_plan {
     long tof_serial = 1;                // bumped by every set_tof
     long adj_tab_serial = 0;            // tof_serial the phase tables were built for
     hipStream_t adj_tab_stream = nullptr;  // stream that built them (reuse is ordered only there)
+    float *d_vsc = nullptr;                // adjoint normalization s / w_n(v) [n_f][n_vox]
+    bool adj_sc_built = false;
+    hipStream_t adj_sc_stream = nullptr;
     long adj_vpr = 0;
     size_t x_bytes = 0, xlp_bytes = 0;
     hipStream_t stream = nullptr;
@@ -316,7 +319,7 @@ void destroy(fn_plan *p)
     if (!p) return;
     hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp); hipFree(p->d_sqf);
     hipFree(p->d_tw); hipFree(p->d_X); hipFree(p->d_Xlp); hipFree(p->d_xscale);
-    hipFree(p->d_delay); hipFree(p->d_P); hipFree(p->d_Z); hipFree(p->d_part); hipFree(p->d_qtab); hipFree(p->d_nw);
+    hipFree(p->d_delay); hipFree(p->d_P); hipFree(p->d_Z); hipFree(p->d_part); hipFree(p->d_qtab); hipFree(p->d_vsc); hipFree(p->d_nw);
     hipFree(p->d_sigF);
     for (auto &r : p->pending) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     for (auto &r : p->spare) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
@@ -860,6 +863,7 @@ int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
         // phase words (128 + 16 n_at per voxel) followed by 4 doubles of aperture terms per voxel
         HIP_CHECK(hipMalloc(&P->d_qtab, (size_t)P->n_vox * (128 + 16 * P->adj_at + 8) * sizeof(uint32_t)),
                   "phase tables");
+        HIP_CHECK(hipMalloc(&P->d_vsc, (size_t)n_f * P->n_vox * sizeof(float)), "adjoint normalization");
     }
     fnk::AdjArgs S{};
     S.B = P->args;
@@ -868,16 +872,23 @@ int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
     S.qrx = P->d_qtab;
     S.qtx = P->d_qtab + (size_t)P->n_vox * 128;
     S.vgeo = reinterpret_cast<const double *>(P->d_qtab + (size_t)P->n_vox * (128 + 16 * P->adj_at));
+    S.vsc = P->d_vsc;
     S.vox_per_range = P->adj_vpr;
     S.n_at = P->adj_at;
     const dim3 grid((unsigned)P->adj_vr, (unsigned)n_chunks, (unsigned)P->adj_at);
     // the phase tables depend on the geometry and the time of flight only: rebuilt
-    // after set_tof, or when this call's stream is not the one that built them
+    // after set_tof, or when this call's stream is not the one that built them; the
+    // normalization table on the geometry only: built once per stream
     const bool tables = P->adj_tab_serial != P->tof_serial || P->adj_tab_stream != s;
-    hipError_t e = fnk::launch_adjoint(P->wcr, tables, grid, s, S);
+    const bool scale = !P->adj_sc_built || P->adj_sc_stream != s;
+    hipError_t e = fnk::launch_adjoint(P->wcr, tables, scale, grid, s, S);
     if (e == hipSuccess && tables) {
         P->adj_tab_serial = P->tof_serial;
         P->adj_tab_stream = s;
+    }
+    if (e == hipSuccess && scale) {
+        P->adj_sc_built = true;
+        P->adj_sc_stream = s;
     }
     if (e != hipSuccess) return hip_fail(e, "adjoint_kernel");
     e = fnk::launch_adjoint_reduce(P->d_P, P->d_Z, P->adj_vr, n_f, P->adj_at, P->n_tx, P->n_el, s);

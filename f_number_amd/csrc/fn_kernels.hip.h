@@ -105,6 +105,7 @@ struct AdjArgs {
     const uint32_t *qrx;    // receive phase words [n_vox][cidx 16][mt 8] (element cidx + 16 mt)
     const uint32_t *qtx;    // transmit phase words [n_vox][16 n_at]
     const double *vgeo;     // per voxel: x, x / pitch, z / (2 pitch), sqrt(z / (2.88 c0))
+    const float *vsc;       // per bin and voxel [n_f][n_vox]: s / w_n(v), 0 outside a valid aperture
     long vox_per_range;     // multiple of 64
     int n_at;               // angle tiles of 16
 };

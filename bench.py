@@ -93,6 +93,21 @@ def cpu_baseline(w, window_name, F_kind, F_params, n_threads):
                       (window_name, 0.0), F, "on", n_threads=n_threads, sel=sel)
     t_col = time.perf_counter() - t0 - t_spec  # das_pw recomputes the spectra
     t_frame = t_spec + max(t_col, 0.0) * w.positions_x.size
+    # one thread: the voxel loop on 32 voxels spread down the column (its spectra stage
+    # on n_threads, timed above and subtracted) and the spectra of 1/16 of the channels
+    n1 = 32
+    iz1 = np.linspace(0, w.positions_z.size - 1, n1).round().astype(int)
+    t0 = time.perf_counter()
+    O.das_pw(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0, w.f_bounds, 0,
+             (window_name, 0.0), F, "on", n_threads=1, n_threads_spectra=n_threads,
+             sel=(np.full(n1, ix_col), iz1))
+    t_vox1 = max(time.perf_counter() - t0 - t_spec, 0.0) / n1
+    ch = rf.reshape(-1, w.n_samples)
+    sub = ch[: max(1, ch.shape[0] // 16)]
+    t0 = time.perf_counter()
+    O.spectra(sub, N, lb, ub, mode=1, n_threads=1)
+    t_spec1 = (time.perf_counter() - t0) * ch.shape[0] / sub.shape[0]
+    t_frame1 = t_spec1 + t_vox1 * w.n_voxels
     return {
         "value": w.n_voxels / t_frame / 1e6,
         "unit": "Mvoxel*frames/s",
@@ -101,6 +116,9 @@ def cpu_baseline(w, window_name, F_kind, F_params, n_threads):
         "sample": f"spectra of all {w.n_tx * w.n_elements} channels + 1 of {w.positions_x.size} lateral "
                   f"columns ({w.positions_z.size} voxels, all {w.n_tx} angles), extrapolated x{w.positions_x.size}; "
                   f"sample wall {t_spec + t_col:.1f} s",
+        "single_thread": {"value": w.n_voxels / t_frame1 / 1e6, "cores": 1,
+                          "sample": f"voxel loop on {n1} voxels of that column and spectra of {sub.shape[0]} of "
+                                    f"{ch.shape[0]} channels, one thread each, extrapolated to the frame"},
     }, np.asarray(col).reshape(-1)
 
 

@@ -509,7 +509,8 @@ static int das_pw_impl(or_das_args_t *a, double c_tof, const double *delay_offse
     const long nch = (long)n_el * a->n_tx;
     double *Xr = (double *)malloc(sizeof(double) * nch * nf);
     double *Xi = (double *)malloc(sizeof(double) * nch * nf);
-    or_spectra(a->rf, a->rf_f32, a->n_t, (int)nch, N, nlb, nub, a->spectra_mode, a->n_threads, Xr, Xi);
+    or_spectra(a->rf, a->rf_f32, a->n_t, (int)nch, N, nlb, nub, a->spectra_mode,
+               a->n_threads_spectra > 0 ? a->n_threads_spectra : a->n_threads, Xr, Xi);
 
     const int clamp = (a->semantics == OR_SEM_CPU);
     const int nvox = a->n_sel > 0 ? a->n_sel : a->n_x * a->n_z;
@@ -744,7 +745,8 @@ int or_das_sa(or_das_args_t *a)
     const long nch = (long)n_el * n_el;
     double *Xr = (double *)malloc(sizeof(double) * nch * nf);
     double *Xi = (double *)malloc(sizeof(double) * nch * nf);
-    or_spectra(a->rf, a->rf_f32, a->n_t, (int)nch, N, nlb, nub, a->spectra_mode, a->n_threads, Xr, Xi);
+    or_spectra(a->rf, a->rf_f32, a->n_t, (int)nch, N, nlb, nub, a->spectra_mode,
+               a->n_threads_spectra > 0 ? a->n_threads_spectra : a->n_threads, Xr, Xi);
 
     const int nvox = a->n_sel > 0 ? a->n_sel : a->n_x * a->n_z;
     /* last voxel the CPU loop processes (das_sa.m:375-498): both apertures valid */

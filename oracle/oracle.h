@@ -95,6 +95,9 @@ typedef struct {
        angles: diverging (z_vs < 0) t_tx = (|r - r_vs| - |r_vs|) / c, focused
        (z_vs > 0) t_tx = (|r_vs| + sign(z - z_vs) |r - r_vs|) / c */
     const double *virtual_sources;
+    /* threads of the spectra stage (0: n_threads); lets a timing run keep the voxel
+       loop single-threaded without a single-threaded DFT of every channel */
+    int n_threads_spectra;
 } or_das_args_t;
 
 /* das_pw.m:302-474, compounded over the n_tx angles (examples.m:70-78). */

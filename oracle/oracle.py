@@ -60,6 +60,7 @@ class _Args(C.Structure):
         ("N_dft", C.c_int), ("n_lb", C.c_int), ("n_ub", C.c_int),
         ("signal_re", C.POINTER(C.c_double)), ("signal_im", C.POINTER(C.c_double)),
         ("virtual_sources", C.POINTER(C.c_double)),
+        ("n_threads_spectra", C.c_int),
     ]
 
 
@@ -181,11 +182,12 @@ def _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics, spect
 def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
            window=("tukey", 0.2), F=None, normalization="on", semantics="cpu",
            spectra_mode=1, n_threads=0, sel=None, norm_window=("boxcar", 0.0),
-           c_tof=None, delay_offsets=None, want_signal=False, virtual_sources=None):
+           c_tof=None, delay_offsets=None, want_signal=False, virtual_sources=None, n_threads_spectra=0):
     """rf: [n_tx, n_el, n_t] C-order (MATLAB [n_t x n_el x n_tx]).  Returns (image, info).
 
     image is [n_z, n_x] complex (or [n_sel] with ``sel=(ix, iz)``), CPU scale.
-    want_signal: info['signal'] = focused RF signal of the last voxel, [N_dft] complex."""
+    want_signal: info['signal'] = focused RF signal of the last voxel, [N_dft] complex.
+    n_threads_spectra: threads of the spectra stage (0: n_threads)."""
     rf = np.asarray(rf)
     if rf.ndim == 2:
         rf = rf[None]
@@ -195,6 +197,7 @@ def das_pw(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
     a, keep, re, im = _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics,
                             spectra_mode, n_threads, sel)
     a.n_t, a.n_el, a.n_tx = n_t, n_el, n_tx
+    a.n_threads_spectra = int(n_threads_spectra)
     a.angles = _dp(angles)
     a.win_rx, a.win_rx_param = WIN[window[0]], float(window[1])
     a.f_rx = F if F is not None else fnumber("angle_lb", 60, 1.5, 10)

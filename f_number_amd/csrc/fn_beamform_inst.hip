@@ -31,11 +31,16 @@ static hipError_t set_lds_once(K kern, size_t lds, size_t &done)
     return e;
 }
 
+// The FP32 kernels stage their spectra tile in static LDS (per-chunk arrays); only
+// the 16-bit forms take it as dynamic LDS of the plan's size.
+static inline size_t dyn_lds(size_t lds) { return INST_PREC == 0 ? 0 : lds; }
+
 template <int MT, int WCR, int WCT>
 static hipError_t launch_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
     auto kern = beamform_kernel<INST_MODE, MT, INST_KI, WCR, WCT, INST_PREC>;
     static size_t lds_done = 0;
+    lds = dyn_lds(lds);
     if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
     hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
     return hipGetLastError();
@@ -63,9 +68,8 @@ template <int MT, int WCR>
 static hipError_t launch_tan_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
     auto kern = beamform_kernel<0, MT, INST_KI, WCR, WC_RANGE, 0, true>;
-    static size_t lds_done = 0;
-    if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
-    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
+    (void)lds;  // FP32: static LDS
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, A);
     return hipGetLastError();
 }
 
@@ -97,6 +101,7 @@ static hipError_t launch_ens_one(dim3 grid, size_t lds, hipStream_t s, const Bea
 {
     auto kern = beamform_kernel<0, MT, INST_KI, WCR, WC_RANGE, INST_PREC, false, true>;
     static size_t lds_done = 0;
+    lds = dyn_lds(lds);
     if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
     hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
     return hipGetLastError();
@@ -178,6 +183,7 @@ static hipError_t launch_persist_one(int grid, size_t lds, hipStream_t s, const 
 {
     auto kern = beamform_persistent_kernel<INST_MODE, MT, INST_KI, WCR, WCT, INST_PREC>;
     static size_t lds_done = 0;
+    lds = dyn_lds(lds);
     if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
     if (grid <= 0) {
         int n = 0;
@@ -235,9 +241,8 @@ template <int MT, int WCR, int WCT>
 static hipError_t launch_nw_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
     auto kern = beamform_kernel<INST_MODE, MT, INST_KI, WCR, WCT, 0, false, false, true>;
-    static size_t lds_done = 0;
-    if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
-    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, A);
+    (void)lds;  // FP32: static LDS
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, A);
     return hipGetLastError();
 }
 

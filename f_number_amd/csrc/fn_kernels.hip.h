@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace fnk {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -445,6 +447,34 @@ struct BeamWavesPerEU {
     static constexpr int value = (MT * KI <= 32 && fits3) ? 3 : WavesPerEU<MT, KI>::value;
 };
 
+// FP32 staging: a bin's spectra tile [MT][KI][64] float2 is held as STAGE_NC
+// chunks of KI / STAGE_NC element steps, each its own LDS object ([MT][KI / STAGE_NC][64]),
+// so that a copy into one chunk (LDS-DMA) and reads of another are disjoint for the
+// compiler: its wait before a read covers only the copies into the chunk read.
+constexpr int STAGE_NC = 4;
+
+// s_waitcnt immediate (gfx9 encoding) that waits for at most n outstanding vector
+// memory operations and leaves the export and LDS/scalar counters unconstrained
+constexpr int waitcnt_vm(int n) { return (n & 0xf) | (((n >> 4) & 0x3) << 14) | (0x7 << 4) | (0xf << 8); }
+
+template <int MT, int KI, int C>
+__device__ __forceinline__ float2 *stage_chunk()
+{
+    __shared__ __align__(16) float2 buf[MT * (KI / STAGE_NC) * 64];
+    return buf;
+}
+
+template <int MT, int KI>
+__device__ __forceinline__ float2 *stage_chunk_at(int c)
+{
+    switch (c) {
+    case 0: return stage_chunk<MT, KI, 0>();
+    case 1: return stage_chunk<MT, KI, 1>();
+    case 2: return stage_chunk<MT, KI, 2>();
+    default: return stage_chunk<MT, KI, 3>();
+    }
+}
+
 // Occupancy target of the persistent batch kernel.  The claim loop around the
 // tile costs registers; left alone the compiler settles small plane-wave tiles
 // (MT x KI <= 64, where LDS admits 3+ workgroups per CU) at 2 waves per SIMD
@@ -552,16 +582,55 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
     constexpr int n_f4 = MT * tile_bytes / 16;
     const int mt_here = min(MT, MT_all - mt0);
 
+    // FP32 staging pipeline: chunk c of bin b is copied by LDS-DMA (global_load_lds_dwordx4,
+    // 1 KB per wave instruction = two element steps of one M-tile) STAGE_NC - 1 chunks
+    // ahead of its contraction, into the chunk array the block finished last, so the
+    // next bin's tile streams in during this bin's contraction.  A chunk is awaited on
+    // this wave's own copies (vmcnt) and one barrier.  Blocks whose group holds fewer
+    // than MT tiles (zero rows to fill), and the windowed form, copy synchronously.
+    static_assert(STAGE_NC == 4, "the awaited counts below assume two chunks in flight behind the current one");
+    constexpr int SJC = KI / STAGE_NC;  // element steps per chunk
+    constexpr bool PIPE = (PREC == PREC_FP32) && !NW && ((MT * KI) % 32 == 0) && (SJC % 2 == 0);
+    constexpr int PIPE_L = PIPE ? MT * KI / 32 : 1;  // 1 KB copies per wave per chunk
+    const bool pipe = PIPE && mt_here == MT;         // block-uniform
+    const int n_chunks = STAGE_NC * (b_end - b_begin);
+    auto issue_chunk = [&](auto cq_const, int bq) {
+        constexpr int cq = decltype(cq_const)::value;
+        const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)bq * MT_all + mt0) * (tile_bytes / 16);
+        float4 *dst = reinterpret_cast<float4 *>(stage_chunk<MT, KI, cq>());
+#pragma unroll
+        for (int i = 0; i < PIPE_L; ++i) {
+            const int u = wave + 4 * i;  // unit: element steps cq SJC + 2 jp, + 1 of M-tile mt
+            const int mt = u / (SJC / 2), jp = u - mt * (SJC / 2);
+            __builtin_amdgcn_global_load_lds(
+                (__attribute__((address_space(1))) const void *)(src + (mt * KI + cq * SJC + 2 * jp) * 32 + lane),
+                (__attribute__((address_space(3))) void *)(dst + (mt * SJC + 2 * jp) * 32), 16, 0, 0);
+        }
+    };
+    if (PREC == PREC_FP32 && pipe) {
+        if (n_chunks > 0) issue_chunk(std::integral_constant<int, 0>{}, b_begin);
+        if (n_chunks > 1) issue_chunk(std::integral_constant<int, 1>{}, b_begin);
+        if (n_chunks > 2) issue_chunk(std::integral_constant<int, 2>{}, b_begin);
+    }
+
     for (int b = b_begin; b < b_end; ++b) {
         const int n = A.n_lb + b;
+        if (PREC == PREC_FP32 && !pipe) {
+            __syncthreads();
+            const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)b * MT_all + mt0) * (tile_bytes / 16);
+            const int n4 = mt_here * (tile_bytes / 16);
+            for (int e = tid; e < n_f4; e += 256) {
+                const int mt = e / (KI * 32), rem = e - mt * (KI * 32), j = rem >> 5;
+                float4 *dst = reinterpret_cast<float4 *>(stage_chunk_at<MT, KI>(j / SJC));
+                dst[(mt * SJC + j % SJC) * 32 + (rem & 31)] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            __syncthreads();
+        }
+        if (PREC != PREC_FP32) {
         __syncthreads();
         {
             float4 *dst = reinterpret_cast<float4 *>(smem_raw);
-            if (PREC == PREC_FP32) {
-                const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)b * MT_all + mt0) * (tile_bytes / 16);
-                const int n4 = mt_here * (tile_bytes / 16);
-                for (int e = tid; e < n_f4; e += 256) dst[e] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
+            {
                 // planes [p][mt][c][64][8] per bin; copy tiles mt0.. of each plane
                 constexpr int pl = Prec<PREC>::planes;
                 constexpr int tile4 = tile_bytes / pl / 16;  // float4 per tile per plane
@@ -575,6 +644,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             }
         }
         __syncthreads();
+        }  // 16-bit staging
 
         double Fr = A.Frx[b];
         if (A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_z);  // das_pw.m:374-377
@@ -600,7 +670,6 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             // Gauss three-multiplication complex product on the matrix cores:
             //   K1 = sum_k Br (Xr + Xi), K2 = sum_k Xr (Bi - Br), K3 = sum_k Xi (Br + Bi)
             //   Re Y = K1 - K3, Im Y = K1 + K2
-            const float2 *xs = reinterpret_cast<const float2 *>(smem_raw);
             floatx4 K1[MT], K2[MT], K3[MT];
             floatx4 T1[TAN ? MT : 1], T2[TAN ? MT : 1], T3[TAN ? MT : 1];
 #pragma unroll
@@ -619,7 +688,34 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             const float ul = ap.inv_wl * A.wrx.s_half, ur = ap.inv_wr * A.wrx.s_half;
             const float ql = ap.inv_wl * A.wrx.q_s, qr = ap.inv_wr * A.wrx.q_s;  // WC_COS_RF slopes
             const int j_hi = min(u_hi >> 2, KI - 1);  // elements 4j + g, g = 0..3
-            for (int j = u_lo >> 2; j <= j_hi; ++j) {
+            auto chunk_step = [&](auto cc_const) {
+            constexpr int cc = decltype(cc_const)::value;
+            if (pipe) {
+                const int q = STAGE_NC * (b - b_begin) + cc;
+                // copies issued after chunk q's: those of chunks q + 1, q + 2 (when they exist)
+                const int later = min(2, n_chunks - 1 - q);
+                if (later >= 2) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * PIPE_L));
+                else if (later == 1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(PIPE_L));
+                else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+                // chunk q landed for every wave; every wave is done with chunk q - 1.  A bare
+                // barrier: the wait above is the only one needed (no release fence, which
+                // would also wait for the copies still in flight); the empty asm keeps the
+                // compiler from moving LDS accesses across it
+                asm volatile("" ::: "memory");
+                __builtin_amdgcn_s_barrier();
+                asm volatile("" ::: "memory");
+                // chunk q + 3 goes where chunk q - 1 was: chunk (cc + 3) % 4 of bin b + (cc + 3) / 4;
+                // the last chunk's copy is issued after its contraction (below), so that
+                // the compiler's own wait before reading the last chunk -- whose copy it
+                // tracks from the start of the bin -- does not also wait for it
+                if (cc != STAGE_NC - 1 && q + STAGE_NC - 1 < n_chunks)
+                    issue_chunk(std::integral_constant<int, (cc + STAGE_NC - 1) % STAGE_NC>{},
+                                b + (cc + STAGE_NC - 1) / STAGE_NC);
+            }
+            const float2 *xs = stage_chunk<MT, KI, cc>();
+            const int j_a = max(u_lo >> 2, cc * SJC), j_b = min(j_hi, cc * SJC + SJC - 1);
+#pragma clang loop unroll(disable)
+            for (int j = j_a; j <= j_b; ++j) {
                 const float d = fmaf((float)j, dstep, dgb);
                 float a;
                 if (WCR == WC_COS_ALL) {
@@ -644,7 +740,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 const float bd = bi - br, bp = br + bi;
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
-                    const float2 xv = xs[(mt * KI + j) * 64 + lane];
+                    const float2 xv = xs[(mt * SJC + (j - cc * SJC)) * 64 + lane];
                     const float xsum = xv.x + xv.y;
                     K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xsum, br, K1[mt], 0, 0, 0);
                     K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.x, bd, K2[mt], 0, 0, 0);
@@ -657,6 +753,14 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     }
                 }
             }
+            if (cc == STAGE_NC - 1 && pipe && STAGE_NC * (b - b_begin) + cc + STAGE_NC - 1 < n_chunks)
+                issue_chunk(std::integral_constant<int, (cc + STAGE_NC - 1) % STAGE_NC>{},
+                            b + (cc + STAGE_NC - 1) / STAGE_NC);
+            };  // chunk_step
+            chunk_step(std::integral_constant<int, 0>{});
+            chunk_step(std::integral_constant<int, 1>{});
+            chunk_step(std::integral_constant<int, 2>{});
+            chunk_step(std::integral_constant<int, 3>{});
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 accR[mt] = K1[mt] - K3[mt];

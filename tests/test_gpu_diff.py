@@ -240,3 +240,44 @@ def test_virtual_source_tangent_and_adjoint_match_oracle(fa, oracle, torch):
     print(f"  image {e[0]:.2e} dI/dc {e[1]:.2e} adjoint {e[2]:.2e}", flush=True)
     plan.close()
     assert max(e) <= 1e-4
+
+
+def test_adjoint_ragged_samples(fa, oracle, torch):
+    """Sample count not a multiple of 4 (the inverse DFT's scalar store path) and a
+    channel count not a multiple of 64 (a partial channel tile): adjoint against the
+    FP64 oracle and the dot-product identity."""
+    from f_number_amd import synth
+
+    n_el, n_ang, n_t = 48, 7, 1021
+    w = synth.scaled(synth.cpwc_small(), 13, 11, n_angles=n_ang, n_samples=n_t, n_speckle=800, n_elements=n_el)
+    w.seed = 7700
+    rf = w.rf_host()
+    delays = np.random.default_rng(7701).normal(0, 20e-9, n_el)
+    print(f"case adjoint ragged n_el={n_el} n_ang={n_ang} n_t={n_t}", flush=True)
+    plan = _plan(fa, w, "hann", "cpu")
+    plan.set_tof(1505.0, delays)
+    rng = np.random.default_rng(7702)
+    y = rng.normal(size=(w.positions_z.size, w.positions_x.size)) + 1j * rng.normal(
+        size=(w.positions_z.size, w.positions_x.size))
+    got = plan.adjoint(_img_dev(torch, y)).cpu().numpy()
+    assert got.shape == (n_ang, n_el, n_t)
+    ref = oracle.das_pw_diff(w.positions_x, w.positions_z, rf, w.f_s, w.angles, w.width, w.pitch, w.c_0,
+                             w.f_bounds, 0.0, **_okw(oracle, "hann", "cpu"), c_tof=1505.0, delay_offsets=delays,
+                             y=y)["adjoint"]
+    e = rel_l2(got, ref)
+    # dot-product identity with y = A x + 0.1 noise (a well-conditioned inner product)
+    x = torch.from_numpy(rf).cuda()
+    Ax = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device="cuda")
+    plan.execute(x, Ax)
+    torch.cuda.synchronize()
+    gen = torch.Generator(device="cuda").manual_seed(7703)
+    noise = torch.randn(Ax.shape, dtype=torch.complex64, device="cuda", generator=gen)
+    yd = (Ax + 0.1 * noise * Ax.abs().mean()).contiguous()
+    AHy = plan.adjoint(yd)
+    torch.cuda.synchronize()
+    lhs = torch.sum((Ax.conj() * yd).real.double()).item()
+    rhs = torch.sum(x.double() * AHy.double()).item()
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"  adjoint rel_l2 {e:.2e}, dot identity rel {rel:.2e}", flush=True)
+    plan.close()
+    assert e <= 1e-4 and rel <= 1e-6

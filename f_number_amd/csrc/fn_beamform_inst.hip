@@ -31,9 +31,10 @@ static hipError_t set_lds_once(K kern, size_t lds, size_t &done)
     return e;
 }
 
-// The FP32 kernels stage their spectra tile in static LDS (per-chunk arrays); only
-// the 16-bit forms take it as dynamic LDS of the plan's size.
-static inline size_t dyn_lds(size_t lds) { return INST_PREC == 0 ? 0 : lds; }
+// The FP32 kernels and the 16-bit ones at KI = 32 stage their spectra tile in static
+// LDS (per-chunk arrays); the 16-bit forms at KI < 32 take it as dynamic LDS of the
+// plan's size.
+static inline size_t dyn_lds(size_t lds) { return (INST_PREC == 0 || INST_KI == 32) ? 0 : lds; }
 
 template <int MT, int WCR, int WCT>
 static hipError_t launch_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)

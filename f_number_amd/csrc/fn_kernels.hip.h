@@ -439,11 +439,13 @@ struct WavesPerEU {
 };
 
 // One-launch-per-plan kernel: the smallest tiles (MT x KI <= 32) are held to 3
-// waves per SIMD where they fit without spilling (not the ensemble epilogue, not
-// the 16-bit forms with the LINEAR or general COS windows); otherwise WavesPerEU.
+// waves per SIMD where they fit without spilling (FP32, fp16, and the split forms
+// with the cosine windows WC_COS_ALL / WC_COS_RF; not the ensemble epilogue);
+// otherwise WavesPerEU.
 template <int MT, int KI, int WCR, int PREC, bool ENS>
 struct BeamWavesPerEU {
-    static constexpr bool fits3 = !ENS && (PREC == PREC_FP32 || (WCR != WC_LINEAR && WCR != WC_COS));
+    static constexpr bool fits3 =
+        !ENS && (PREC == PREC_FP32 || PREC == PREC_FP16 || WCR == WC_COS_ALL || WCR == WC_COS_RF);
     static constexpr int value = (MT * KI <= 32 && fits3) ? 3 : WavesPerEU<MT, KI>::value;
 };
 
@@ -464,6 +466,25 @@ __device__ __forceinline__ float2 *stage_chunk()
     return buf;
 }
 
+// 16-bit forms (KI = 32): chunk c holds [plane][M-tile][64] 16-byte groups of eight halves
+template <int PREC, int MT, int C>
+__device__ __forceinline__ u32x4 *stage_chunk_lp()
+{
+    __shared__ __align__(16) u32x4 buf[Prec<PREC>::planes * MT * 64];
+    return buf;
+}
+
+template <int PREC, int MT>
+__device__ __forceinline__ u32x4 *stage_chunk_lp_at(int c)
+{
+    switch (c) {
+    case 0: return stage_chunk_lp<PREC, MT, 0>();
+    case 1: return stage_chunk_lp<PREC, MT, 1>();
+    case 2: return stage_chunk_lp<PREC, MT, 2>();
+    default: return stage_chunk_lp<PREC, MT, 3>();
+    }
+}
+
 template <int MT, int KI>
 __device__ __forceinline__ float2 *stage_chunk_at(int c)
 {
@@ -479,11 +500,12 @@ __device__ __forceinline__ float2 *stage_chunk_at(int c)
 // tile costs registers; left alone the compiler settles small plane-wave tiles
 // (MT x KI <= 64, where LDS admits 3+ workgroups per CU) at 2 waves per SIMD
 // where the one-launch-per-plan kernel reaches 3.  A target of 3 is stated where
-// the tile still fits without spilling: the 16-bit forms except the LINEAR and
-// general COS windows, and FP32 with the full-array Hann (WC_COS_ALL).
+// the tile still fits without spilling: one transmit tile in the 16-bit forms (fp16,
+// and the split forms with the cosine windows WC_COS_ALL / WC_COS_RF).
 template <int MODE, int MT, int KI, int WCR, int PREC>
 struct PersistWavesPerEU {
-    static constexpr bool fits3 = (PREC == PREC_FP32) ? (WCR == WC_COS_ALL) : (WCR != WC_LINEAR && WCR != WC_COS);
+    static constexpr bool fits3 =
+        MT == 1 && (PREC == PREC_FP16 || (PREC != PREC_FP32 && (WCR == WC_COS_ALL || WCR == WC_COS_RF)));
     static constexpr int value = (MODE == MODE_PW && MT <= 3 && MT * KI <= 64 && fits3) ? 3 : WavesPerEU<MT, KI>::value;
 };
 
@@ -582,69 +604,133 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
     constexpr int n_f4 = MT * tile_bytes / 16;
     const int mt_here = min(MT, MT_all - mt0);
 
-    // FP32 staging pipeline: chunk c of bin b is copied by LDS-DMA (global_load_lds_dwordx4,
-    // 1 KB per wave instruction = two element steps of one M-tile) STAGE_NC - 1 chunks
-    // ahead of its contraction, into the chunk array the block finished last, so the
-    // next bin's tile streams in during this bin's contraction.  A chunk is awaited on
-    // this wave's own copies (vmcnt) and one barrier.  Blocks whose group holds fewer
-    // than MT tiles (zero rows to fill), and the windowed form, copy synchronously.
+    // Staging pipeline: chunk c of bin b is copied by LDS-DMA (global_load_lds_dwordx4,
+    // 1 KB per wave instruction: two element steps of one M-tile in FP32, one plane of
+    // one M-tile over 32 elements in the 16-bit forms) STAGE_NC - 1 chunks ahead of its
+    // contraction, into the chunk array the block finished last, so the next bin's
+    // tile streams in during this bin's contraction.  A chunk is awaited on this wave's
+    // own copies (vmcnt) and one barrier.  Blocks whose group holds fewer than MT tiles
+    // (zero rows to fill), the windowed form, and tilings whose copies do not divide
+    // evenly over the four waves copy synchronously; the 16-bit forms at KI < 32 keep
+    // one dynamic LDS tile.
     static_assert(STAGE_NC == 4, "the awaited counts below assume two chunks in flight behind the current one");
-    constexpr int SJC = KI / STAGE_NC;  // element steps per chunk
-    constexpr bool PIPE = (PREC == PREC_FP32) && !NW && ((MT * KI) % 32 == 0) && (SJC % 2 == 0);
-    constexpr int PIPE_L = PIPE ? MT * KI / 32 : 1;  // 1 KB copies per wave per chunk
+    constexpr int SJC = KI / STAGE_NC;  // FP32 element steps per chunk
+    constexpr int KC = KI / 8;          // 16-bit: chunks of 32 elements
+    constexpr int PL = Prec<PREC>::planes;
+    constexpr bool LP_CHUNKED = (PREC != PREC_FP32) && KC == STAGE_NC;
+    constexpr bool CHUNKED = (PREC == PREC_FP32) || LP_CHUNKED;
+    constexpr int UNITS = (PREC == PREC_FP32) ? MT * KI / 8 : PL * MT;  // 1 KB copies per chunk
+    constexpr bool PIPE = CHUNKED && !NW && (PREC != PREC_FP32 || SJC % 2 == 0);
+    // 1 KB copies per wave per chunk: PIPE_LMAX for the first UNITS % 4 waves, PIPE_L
+    // for the others; the awaited counts use PIPE_L, which no wave exceeds in copies
+    // issued after the awaited chunk (so every wait covers at least that chunk)
+    constexpr int PIPE_L = UNITS / 4, PIPE_LMAX = (UNITS + 3) / 4;
     const bool pipe = PIPE && mt_here == MT;         // block-uniform
     const int n_chunks = STAGE_NC * (b_end - b_begin);
     auto issue_chunk = [&](auto cq_const, int bq) {
         constexpr int cq = decltype(cq_const)::value;
-        const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)bq * MT_all + mt0) * (tile_bytes / 16);
-        float4 *dst = reinterpret_cast<float4 *>(stage_chunk<MT, KI, cq>());
+        if constexpr (PREC == PREC_FP32) {
+            const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)bq * MT_all + mt0) * (tile_bytes / 16);
+            float4 *dst = reinterpret_cast<float4 *>(stage_chunk<MT, KI, cq>());
 #pragma unroll
-        for (int i = 0; i < PIPE_L; ++i) {
-            const int u = wave + 4 * i;  // unit: element steps cq SJC + 2 jp, + 1 of M-tile mt
-            const int mt = u / (SJC / 2), jp = u - mt * (SJC / 2);
-            __builtin_amdgcn_global_load_lds(
-                (__attribute__((address_space(1))) const void *)(src + (mt * KI + cq * SJC + 2 * jp) * 32 + lane),
-                (__attribute__((address_space(3))) void *)(dst + (mt * SJC + 2 * jp) * 32), 16, 0, 0);
+            for (int i = 0; i < PIPE_LMAX; ++i) {
+                const int u = wave + 4 * i;  // unit: element steps cq SJC + 2 jp, + 1 of M-tile mt
+                if (PIPE_LMAX != PIPE_L && u >= UNITS) break;  // wave-uniform
+                const int mt = u / (SJC / 2), jp = u - mt * (SJC / 2);
+                __builtin_amdgcn_global_load_lds(
+                    (__attribute__((address_space(1))) const void *)(src + (mt * KI + cq * SJC + 2 * jp) * 32 + lane),
+                    (__attribute__((address_space(3))) void *)(dst + (mt * SJC + 2 * jp) * 32), 16, 0, 0);
+            }
+        } else if constexpr (LP_CHUNKED) {
+            // planes [p][mt][c][64] of 16-byte groups per bin; unit (p, mt) of chunk cq
+            const u32x4 *src = reinterpret_cast<const u32x4 *>(A.Xlp) + (size_t)bq * PL * MT_all * KC * 64;
+            u32x4 *dst = stage_chunk_lp<PREC, MT, cq>();
+#pragma unroll
+            for (int i = 0; i < PIPE_LMAX; ++i) {
+                const int u = wave + 4 * i;
+                if (PIPE_LMAX != PIPE_L && u >= UNITS) break;  // wave-uniform
+                const int pp = u / MT, mt = u - pp * MT;
+                __builtin_amdgcn_global_load_lds(
+                    (__attribute__((address_space(1))) const void *)(src + ((pp * MT_all + mt0 + mt) * KC + cq) * 64 + lane),
+                    (__attribute__((address_space(3))) void *)(dst + (pp * MT + mt) * 64), 16, 0, 0);
+            }
         }
     };
-    if (PREC == PREC_FP32 && pipe) {
+    if (pipe) {
         if (n_chunks > 0) issue_chunk(std::integral_constant<int, 0>{}, b_begin);
         if (n_chunks > 1) issue_chunk(std::integral_constant<int, 1>{}, b_begin);
         if (n_chunks > 2) issue_chunk(std::integral_constant<int, 2>{}, b_begin);
     }
+    // top of a chunk's contraction: await chunk q of the sequence, then hand the
+    // positions of chunk q - 1 to chunk q + 3 (the last chunk of a bin hands them
+    // over after its contraction, see below)
+    auto chunk_begin = [&](auto cc_const, int b) {
+        constexpr int cc = decltype(cc_const)::value;
+        const int q = STAGE_NC * (b - b_begin) + cc;
+        // copies issued after chunk q's: those of chunks q + 1, q + 2 (when they exist)
+        const int later = min(2, n_chunks - 1 - q);
+        if (later >= 2) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * PIPE_L));
+        else if (later == 1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(PIPE_L));
+        else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+        // chunk q landed for every wave; every wave is done with chunk q - 1.  A bare
+        // barrier: the wait above is the only one needed (no release fence, which
+        // would also wait for the copies still in flight); the empty asm keeps the
+        // compiler from moving LDS accesses across it
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        // chunk q + 3 goes where chunk q - 1 was: chunk (cc + 3) % 4 of bin b + (cc + 3) / 4;
+        // the last chunk's copy is issued after its contraction, so that the compiler's
+        // own wait before reading the last chunk -- whose copy it tracks from the start
+        // of the bin -- does not also wait for it
+        if (cc != STAGE_NC - 1 && q + STAGE_NC - 1 < n_chunks)
+            issue_chunk(std::integral_constant<int, (cc + STAGE_NC - 1) % STAGE_NC>{}, b + (cc + STAGE_NC - 1) / STAGE_NC);
+    };
+    auto chunk_end = [&](auto cc_const, int b) {
+        constexpr int cc = decltype(cc_const)::value;
+        if (cc == STAGE_NC - 1 && STAGE_NC * (b - b_begin) + cc + STAGE_NC - 1 < n_chunks)
+            issue_chunk(std::integral_constant<int, (cc + STAGE_NC - 1) % STAGE_NC>{}, b + (cc + STAGE_NC - 1) / STAGE_NC);
+    };
 
     for (int b = b_begin; b < b_end; ++b) {
         const int n = A.n_lb + b;
-        if (PREC == PREC_FP32 && !pipe) {
+        if (CHUNKED && !pipe) {
             __syncthreads();
-            const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)b * MT_all + mt0) * (tile_bytes / 16);
-            const int n4 = mt_here * (tile_bytes / 16);
-            for (int e = tid; e < n_f4; e += 256) {
-                const int mt = e / (KI * 32), rem = e - mt * (KI * 32), j = rem >> 5;
-                float4 *dst = reinterpret_cast<float4 *>(stage_chunk_at<MT, KI>(j / SJC));
-                dst[(mt * SJC + j % SJC) * 32 + (rem & 31)] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            __syncthreads();
-        }
-        if (PREC != PREC_FP32) {
-        __syncthreads();
-        {
-            float4 *dst = reinterpret_cast<float4 *>(smem_raw);
-            {
-                // planes [p][mt][c][64][8] per bin; copy tiles mt0.. of each plane
-                constexpr int pl = Prec<PREC>::planes;
-                constexpr int tile4 = tile_bytes / pl / 16;  // float4 per tile per plane
-                const float4 *src = reinterpret_cast<const float4 *>(A.Xlp) + (size_t)b * pl * MT_all * tile4;
+            if constexpr (PREC == PREC_FP32) {
+                const float4 *src = reinterpret_cast<const float4 *>(A.X) + ((size_t)b * MT_all + mt0) * (tile_bytes / 16);
+                const int n4 = mt_here * (tile_bytes / 16);
                 for (int e = tid; e < n_f4; e += 256) {
-                    const int p = e / (MT * tile4), rem = e - p * (MT * tile4);
-                    const int mt = rem / tile4;
-                    dst[e] = (mt < mt_here) ? src[((size_t)p * MT_all + mt0 + mt) * tile4 + (rem - mt * tile4)]
-                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const int mt = e / (KI * 32), rem = e - mt * (KI * 32), j = rem >> 5;
+                    float4 *dst = reinterpret_cast<float4 *>(stage_chunk_at<MT, KI>(j / SJC));
+                    dst[(mt * SJC + j % SJC) * 32 + (rem & 31)] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            } else if constexpr (LP_CHUNKED) {
+                constexpr int tile4 = KC * 64;  // 16-byte groups per tile per plane
+                const u32x4 *src = reinterpret_cast<const u32x4 *>(A.Xlp) + (size_t)b * PL * MT_all * tile4;
+                for (int e = tid; e < PL * MT * tile4; e += 256) {
+                    const int pp = e / (MT * tile4), rem = e - pp * (MT * tile4);
+                    const int mt = rem / tile4, r2 = rem - mt * tile4, cq = r2 >> 6;
+                    u32x4 *dst = stage_chunk_lp_at<PREC, MT>(cq);
+                    dst[(pp * MT + mt) * 64 + (r2 & 63)] =
+                        (mt < mt_here) ? src[((size_t)pp * MT_all + mt0 + mt) * tile4 + r2] : u32x4{0u, 0u, 0u, 0u};
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-        }  // 16-bit staging
+        if (!CHUNKED) {
+            // 16-bit planes [p][mt][c][64][8] per bin in one dynamic LDS tile
+            __syncthreads();
+            float4 *dst = reinterpret_cast<float4 *>(smem_raw);
+            constexpr int tile4 = tile_bytes / PL / 16;  // float4 per tile per plane
+            const float4 *src = reinterpret_cast<const float4 *>(A.Xlp) + (size_t)b * PL * MT_all * tile4;
+            for (int e = tid; e < n_f4; e += 256) {
+                const int pp = e / (MT * tile4), rem = e - pp * (MT * tile4);
+                const int mt = rem / tile4;
+                dst[e] = (mt < mt_here) ? src[((size_t)pp * MT_all + mt0 + mt) * tile4 + (rem - mt * tile4)]
+                                        : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            __syncthreads();
+        }
 
         double Fr = A.Frx[b];
         if (A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_z);  // das_pw.m:374-377
@@ -666,7 +752,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
         // recompute x_k - x per bin (one FMA each) instead of holding KI registers
         float dgb = dg;
         asm volatile("" : "+v"(dgb));
-        if (PREC == PREC_FP32) {
+        if constexpr (PREC == PREC_FP32) {
             // Gauss three-multiplication complex product on the matrix cores:
             //   K1 = sum_k Br (Xr + Xi), K2 = sum_k Xr (Bi - Br), K3 = sum_k Xi (Br + Bi)
             //   Re Y = K1 - K3, Im Y = K1 + K2
@@ -690,28 +776,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             const int j_hi = min(u_hi >> 2, KI - 1);  // elements 4j + g, g = 0..3
             auto chunk_step = [&](auto cc_const) {
             constexpr int cc = decltype(cc_const)::value;
-            if (pipe) {
-                const int q = STAGE_NC * (b - b_begin) + cc;
-                // copies issued after chunk q's: those of chunks q + 1, q + 2 (when they exist)
-                const int later = min(2, n_chunks - 1 - q);
-                if (later >= 2) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * PIPE_L));
-                else if (later == 1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(PIPE_L));
-                else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-                // chunk q landed for every wave; every wave is done with chunk q - 1.  A bare
-                // barrier: the wait above is the only one needed (no release fence, which
-                // would also wait for the copies still in flight); the empty asm keeps the
-                // compiler from moving LDS accesses across it
-                asm volatile("" ::: "memory");
-                __builtin_amdgcn_s_barrier();
-                asm volatile("" ::: "memory");
-                // chunk q + 3 goes where chunk q - 1 was: chunk (cc + 3) % 4 of bin b + (cc + 3) / 4;
-                // the last chunk's copy is issued after its contraction (below), so that
-                // the compiler's own wait before reading the last chunk -- whose copy it
-                // tracks from the start of the bin -- does not also wait for it
-                if (cc != STAGE_NC - 1 && q + STAGE_NC - 1 < n_chunks)
-                    issue_chunk(std::integral_constant<int, (cc + STAGE_NC - 1) % STAGE_NC>{},
-                                b + (cc + STAGE_NC - 1) / STAGE_NC);
-            }
+            if (pipe) chunk_begin(cc_const, b);
             const float2 *xs = stage_chunk<MT, KI, cc>();
             const int j_a = max(u_lo >> 2, cc * SJC), j_b = min(j_hi, cc * SJC + SJC - 1);
 #pragma clang loop unroll(disable)
@@ -753,9 +818,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     }
                 }
             }
-            if (cc == STAGE_NC - 1 && pipe && STAGE_NC * (b - b_begin) + cc + STAGE_NC - 1 < n_chunks)
-                issue_chunk(std::integral_constant<int, (cc + STAGE_NC - 1) % STAGE_NC>{},
-                            b + (cc + STAGE_NC - 1) / STAGE_NC);
+            if (pipe) chunk_end(cc_const, b);
             };  // chunk_step
             chunk_step(std::integral_constant<int, 0>{});
             chunk_step(std::integral_constant<int, 1>{});
@@ -774,9 +837,10 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
         } else {
             // 16-bit operands, FP32 accumulation: Re Y = Xr Br - Xi Bi, Im Y = Xr Bi + Xi Br;
             // split forms add hi*lo + lo*hi passes (x = hi + lo)
-            constexpr int KC = KI / 8;
             constexpr int pl = Prec<PREC>::planes;
-            const u32x4 *xq = reinterpret_cast<const u32x4 *>(smem_raw);  // [p][mt][c][64]
+            // (p, mt) group of chunk cc: per-chunk arrays [p][mt][64] (KI = 32) or the dynamic
+            // tile [p][mt][c][64]
+            constexpr int XS = LP_CHUNKED ? 64 : KC * 64;
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 accR[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -784,9 +848,13 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             }
             const float ul = ap.inv_wl * A.wrx.s_half, ur = ap.inv_wr * A.wrx.s_half;
             const float ql = ap.inv_wl * A.wrx.q_s, qr = ap.inv_wr * A.wrx.q_s;  // WC_COS_RF slopes
-#pragma unroll
-            for (int cc = 0; cc < KC; ++cc) {
-                if (cc < (u_lo >> 5) || cc > (u_hi >> 5)) continue;  // elements 32 cc .. 32 cc + 31
+            auto chunk_lp = [&](auto cc_const) {
+                constexpr int cc = decltype(cc_const)::value;
+                if (pipe) chunk_begin(cc_const, b);
+                const u32x4 *xq;
+                if constexpr (LP_CHUNKED) xq = stage_chunk_lp<PREC, MT, cc>();
+                else xq = reinterpret_cast<const u32x4 *>(smem_raw) + cc * 64;
+                if (cc >= (u_lo >> 5) && cc <= (u_hi >> 5)) {  // elements 32 cc .. 32 cc + 31
                 float br[8], bi[8];
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {
@@ -817,15 +885,15 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 const u32x4 nbih = neg16(bih);
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
-                    const u32x4 xrh = xq[((0 * MT + mt) * KC + cc) * 64 + lane];
-                    const u32x4 xih = xq[((1 * MT + mt) * KC + cc) * 64 + lane];
+                    const u32x4 xrh = xq[(0 * MT + mt) * XS + lane];
+                    const u32x4 xih = xq[(1 * MT + mt) * XS + lane];
                     accR[mt] = mfma16<PREC>(xrh, brh, accR[mt]);
                     accR[mt] = mfma16<PREC>(xih, nbih, accR[mt]);
                     accI[mt] = mfma16<PREC>(xrh, bih, accI[mt]);
                     accI[mt] = mfma16<PREC>(xih, brh, accI[mt]);
                     if (Prec<PREC>::split) {
-                        const u32x4 xrl = xq[((2 * MT + mt) * KC + cc) * 64 + lane];
-                        const u32x4 xil = xq[((3 * MT + mt) * KC + cc) * 64 + lane];
+                        const u32x4 xrl = xq[(2 * MT + mt) * XS + lane];
+                        const u32x4 xil = xq[(3 * MT + mt) * XS + lane];
                         const u32x4 nbil = neg16(bil);
                         accR[mt] = mfma16<PREC>(xrh, brl, accR[mt]);
                         accR[mt] = mfma16<PREC>(xrl, brh, accR[mt]);
@@ -837,6 +905,14 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                         accI[mt] = mfma16<PREC>(xil, brh, accI[mt]);
                     }
                 }
+                }  // chunk in the support
+                if (pipe) chunk_end(cc_const, b);
+            };  // chunk_lp
+            chunk_lp(std::integral_constant<int, 0>{});
+            if constexpr (KC > 1) chunk_lp(std::integral_constant<int, 1>{});
+            if constexpr (KC > 2) {
+                chunk_lp(std::integral_constant<int, 2>{});
+                chunk_lp(std::integral_constant<int, 3>{});
             }
             (void)pl;
         }

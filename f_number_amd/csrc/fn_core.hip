@@ -1205,7 +1205,7 @@ struct BatchClass {
 struct fn_batch {
     std::vector<fn_plan *> plans;
     int device = 0;
-    int persistent = 1;                // plans sharing a kernel class run as one persistent launch
+    int persistent = 0;                // plans sharing a kernel class run as one persistent launch
     std::vector<BatchClass> classes;
     std::vector<int> singles;          // plans launched on their own
     hipStream_t pstream = nullptr;     // stream of the persistent launches inside the capture

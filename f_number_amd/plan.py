@@ -256,7 +256,7 @@ class Batch:
     replayed by one graph launch per execute(); a change of any buffer pointer
     captures again."""
 
-    def __init__(self, plans, n_streams: int = 4, persistent: bool = True):
+    def __init__(self, plans, n_streams: int = 4, persistent: bool = False):
         self.plans = list(plans)
         arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
         self._h = C.c_void_p()

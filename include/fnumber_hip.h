@@ -270,7 +270,8 @@ int32_t fn_batch_execute(fn_batch *batch, const void *const *d_rf, const int32_t
  * longest chunks first.  A class is cut into launches of at most 8 plans, each
  * waiting only for its own plans' spectra.  enable = 0 launches every plan on
  * its own (each still a graph node); the next execute captures again.
- * Default: enabled. */
+ * Default: disabled (one launch per plan measured faster on the mixed
+ * ensemble + job-queue batch since the pipelined tile staging). */
 int32_t fn_batch_set_persistent(fn_batch *batch, int32_t enable);
 /* graph nodes and the number of captures so far */
 int32_t fn_batch_info(const fn_batch *batch, int64_t *n_nodes, int32_t *n_captures);

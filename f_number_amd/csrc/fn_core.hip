@@ -482,7 +482,8 @@ int launch_spectra(fn_plan *P, const void *d_rf, int dtype, hipStream_t s)
 {
     P->executed = 1;
     const fnk::SpecArgs &S = P->sargs;
-    dim3 sg((unsigned)((S.n_f + 63) / 64), (unsigned)((S.n_ch + 63) / 64));
+    // 64 bins x one operand group (16 rows x 4 elements) per block
+    dim3 sg((unsigned)((S.n_f + 63) / 64), (unsigned)(S.MT * S.KI));
     static size_t done_f = 0, done_d = 0;
     HIP_CHECK(dtype == FN_DTYPE_F32
                   ? lds_limit(reinterpret_cast<const void *>(fnk::spectra_kernel<float>), P->lds_spec, done_f)

@@ -286,72 +286,74 @@ __host__ __device__ __forceinline__ double pw_tx_time(const BeamArgs &A, int a, 
 // -------------------------------------------------------------------------
 // band-limited DFT
 // -------------------------------------------------------------------------
-// block: 256 threads, tile 64 bins x 64 channels, sample chunks of 64.
-// thread: 4 bins (stride 16) x 4 channels (stride 16).
+// Band-limited DFT on the matrix cores: for every bin n and channel ch,
+//   X_n(ch) = sum_t x_ch[t] e^{-j 2 pi n t / N}
+// as two real products D[bin][ch] += W[bin][t] R[t][ch] (W = cos, then -sin) on
+// v_mfma_f32_16x16x4f32, K = samples.  A block (4 waves) owns 64 bins and one channel
+// group of the operand layout -- 16 transmit rows a = 16 A .. 16 A + 15 times 4 receive
+// elements k = 4 K4 .. 4 K4 + 3 -- so that each bin's output is one contiguous 64-value
+// block ((b MT + A) KI + K4) 64 + (a & 15) + 16 (k & 3).  Wave w holds bins 16 w ..
+// 16 w + 15 of the block (MFMA M) and the group's 64 channels as 4 N-tiles (N-tile j =
+// element 4 K4 + j, column i = row 16 A + i).  Lane l supplies W[bin l % 16][t0 + l / 16]
+// from the LDS twiddle table at the exactly advanced index (n t) mod N, and
+// R[t0 + l / 16][channel] from the RF tile staged in LDS.  Rows and elements of the
+// padded operand layout get zeros.  Block grid: (ceil(n_f / 64), MT KI).
 template <typename T>
 __global__ __launch_bounds__(256) void spectra_kernel(const T *__restrict__ rf, const float2 *__restrict__ twiddle,
-                                                      float2 *__restrict__ X, SpecArgs S)
+                                                           float2 *__restrict__ X, SpecArgs S)
 {
     extern __shared__ float smem[];
-    float2 *tw = reinterpret_cast<float2 *>(smem);       // [N]
-    float *xs = smem + 2 * S.N;                          // [64][65]
-    const int tid = threadIdx.x;
-    const int bi = tid & 15, cg = tid >> 4;
-    const int b0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    float2 *tw = reinterpret_cast<float2 *>(smem);  // [N]
+    float *xs = smem + 2 * S.N;                     // [64 channels][65]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, q = lane >> 4;
+    const int grp = blockIdx.y, A = grp / S.KI, K4 = grp - A * S.KI;
+    const int b0 = blockIdx.x * 64;
     for (int m = tid; m < S.N; m += 256) tw[m] = twiddle[m];
 
-    int nq[4], mq[4];
-    float ar[4][4], ai[4][4];
+    const int b_l = b0 + 16 * wave + r16;                  // this lane's bin (A operand row)
+    const int n = (b_l < S.n_f) ? S.n_lb + b_l : 0;        // bins past the band are computed as n = 0 and dropped
+    const int dm = (int)((4LL * n) % S.N);                 // index step per K step (4 samples)
+    floatx4 accR[4], accI[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int b = b0 + bi + 16 * q;
-        nq[q] = (b < S.n_f) ? S.n_lb + b : 0;  // bins past the band are computed as n = 0 and dropped
-        mq[q] = 0;
-#pragma unroll
-        for (int p = 0; p < 4; ++p) { ar[q][p] = 0.f; ai[q][p] = 0.f; }
+    for (int j = 0; j < 4; ++j) {
+        accR[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        accI[j] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
     for (int t0 = 0; t0 < S.n_t; t0 += 64) {
         __syncthreads();
         for (int e = tid; e < 64 * 64; e += 256) {
-            const int cl = e >> 6, tl = e & 63;
-            const int ch = c0 + cl, t = t0 + tl;
+            const int cl = e >> 6, tl = e & 63;          // channel of the group, sample of the chunk
+            const int a = 16 * A + (cl & 15), k = 4 * K4 + (cl >> 4);
+            const int t = t0 + tl;
             float v = 0.f;
-            if (ch < S.n_ch && t < S.n_t) v = (float)rf[(size_t)ch * S.n_t + t];
+            if (a < S.n_tx && k < S.n_el && t < S.n_t) v = (float)rf[((size_t)a * S.n_el + k) * S.n_t + t];
             xs[cl * 65 + tl] = v;
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) mq[q] = (int)(((int64_t)nq[q] * t0) % S.N);
         __syncthreads();
-        const int tn = min(64, S.n_t - t0);
-        for (int tl = 0; tl < tn; ++tl) {
-            float xv[4];
+        int m = (int)(((int64_t)n * (t0 + q)) % S.N);
+        const int n_steps = (min(64, S.n_t - t0) + 3) >> 2;  // samples past n_t are staged as zeros
+        for (int st = 0; st < n_steps; ++st) {
+            const float2 w = tw[m];
+            m += dm;
+            if (m >= S.N) m -= S.N;
+            const int tl = 4 * st + q;
 #pragma unroll
-            for (int p = 0; p < 4; ++p) xv[p] = xs[(cg + 16 * p) * 65 + tl];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float2 w = tw[mq[q]];
-#pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    ar[q][p] = fmaf(xv[p], w.x, ar[q][p]);
-                    ai[q][p] = fmaf(-xv[p], w.y, ai[q][p]);
-                }
-                int m = mq[q] + nq[q];
-                mq[q] = (m >= S.N) ? m - S.N : m;
+            for (int j = 0; j < 4; ++j) {
+                const float xv = xs[(16 * j + r16) * 65 + tl];
+                accR[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x, xv, accR[j], 0, 0, 0);
+                accI[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-w.y, xv, accI[j], 0, 0, 0);
             }
         }
     }
+    // lane holds D[bin 16 wave + 4 q + r][column r16] of N-tile j
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int b = b0 + bi + 16 * q;
+    for (int r = 0; r < 4; ++r) {
+        const int b = b0 + 16 * wave + 4 * q + r;
         if (b >= S.n_f) continue;
+        float2 *dst = X + (((size_t)b * S.MT + A) * S.KI + K4) * 64 + r16;
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const int ch = c0 + cg + 16 * p;
-            if (ch >= S.n_ch) continue;
-            const int a = ch / S.n_el, k = ch - a * S.n_el;
-            const size_t off = (((size_t)b * S.MT + (a >> 4)) * S.KI + (k >> 2)) * 64 + (a & 15) + 16 * (k & 3);
-            X[off] = make_float2(ar[q][p], ai[q][p]);
-        }
+        for (int j = 0; j < 4; ++j) dst[16 * j] = make_float2(accR[j][r], accI[j][r]);
     }
 }
 

@@ -9,6 +9,7 @@ from ._lib import FNumberError, LIB_PATH, load  # noqa: F401
 from .das import das_pw, das_sa  # noqa: F401
 from .plan import Batch, PwPlan, SaPlan  # noqa: F401
 from . import autofocus, examples, synth, tuning  # noqa: F401,E402
+from .linescan import LineScan, das_lbl  # noqa: F401,E402
 
 __all__ = ["das_pw", "das_sa", "PwPlan", "SaPlan", "windows", "f_numbers", "normalizations", "platforms",
-           "FNumberError", "load"]
+           "FNumberError", "load", "Batch", "LineScan", "das_lbl"]

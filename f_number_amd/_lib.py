@@ -48,7 +48,7 @@ class PwParams(C.Structure):
         ("normalization", C.c_int32), ("normalization_window", Window),
         ("semantics", C.c_int32), ("precision", C.c_int32), ("output_scale", C.c_int32),
         ("index_gpu", C.c_int32), ("verbosity", C.c_int32), ("n_frames", C.c_int32),
-        ("virtual_sources", _dp),
+        ("virtual_sources", _dp), ("line_scan", C.c_int32),
     ]
 
 

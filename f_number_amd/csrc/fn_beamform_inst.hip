@@ -92,6 +92,27 @@ hipError_t FN_TAN_LAUNCHER(INST_KI)(int mt, int wcr, dim3 grid, size_t lds, hipS
     }
     return hipErrorInvalidValue;
 }
+
+// line scan: block z = grid column = its transmit row, one transmit tile
+#define FN_LBL_LAUNCHER(k) FN_CAT4(launch_beamform_lbl, _k, k, )
+template <int WCR>
+static hipError_t launch_lbl_one(dim3 grid, hipStream_t s, const BeamArgs &A)
+{
+    auto kern = beamform_kernel<0, 1, INST_KI, WCR, WC_RANGE, 0, false, false, false, true>;
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, A);  // FP32: static LDS
+    return hipGetLastError();
+}
+
+hipError_t FN_LBL_LAUNCHER(INST_KI)(int wcr, dim3 grid, hipStream_t s, const BeamArgs &A)
+{
+    switch (wcr) {
+    case WC_RANGE: return launch_lbl_one<WC_RANGE>(grid, s, A);
+    case WC_LINEAR: return launch_lbl_one<WC_LINEAR>(grid, s, A);
+    case WC_COS_ALL: return launch_lbl_one<WC_COS_ALL>(grid, s, A);
+    case WC_COS_RF: return launch_lbl_one<WC_COS_RF>(grid, s, A);
+    default: return launch_lbl_one<WC_COS>(grid, s, A);
+    }
+}
 #endif
 
 #if INST_MODE == 0

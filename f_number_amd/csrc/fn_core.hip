@@ -75,6 +75,11 @@ hipError_t launch_focused_signal(const BeamArgs &A, int mode, int wcr, int wct, 
 hipError_t launch_beamform_tan_k8(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k16(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
 hipError_t launch_beamform_tan_k32(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_lbl_k8(int, dim3, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_lbl_k16(int, dim3, hipStream_t, const BeamArgs &);
+hipError_t launch_beamform_lbl_k32(int, dim3, hipStream_t, const BeamArgs &);
+typedef hipError_t (*LblLauncher)(int, dim3, hipStream_t, const BeamArgs &);
+static const LblLauncher kLblLaunchers[3] = {launch_beamform_lbl_k8, launch_beamform_lbl_k16, launch_beamform_lbl_k32};
 hipError_t launch_adjoint(int wcr, bool tables, bool scale, dim3 grid, hipStream_t s, const AdjArgs &S);
 hipError_t launch_adjoint_reduce(const float *P, float2 *Z, int n_vr, int n_f, int n_at, int n_tx, int n_el,
                                  hipStream_t s);
@@ -126,6 +131,7 @@ struct fn_plan {
     int n_frames = 1;  // ensemble frames (transmit rows = frames x angles)
     int wcr = 0, wct = 0;
     int nw = 0;                 // windowed normalization (CPU semantics): taper kernels, one bin chunk
+    int lbl = 0;                // line scan: grid column i from transmit row i (groups = columns)
     float *d_nw = nullptr;      // L / 2 per (voxel, element): receive, then (SA) transmit
     int executed = 0;           // spectra of an execute are in d_X (focused signal)
     float2 *d_sigF = nullptr;   // focused spectrum of the signal voxel [n_f]
@@ -163,6 +169,9 @@ struct fn_plan {
 };
 
 namespace {
+
+// voxel tiles of 64 per beamforming group: the whole grid, or (line scan) one column
+long beam_tiles(const fn_plan *P) { return P->lbl ? (P->n_z + 63) / 64 : (P->n_vox + 63) / 64; }
 
 int hip_fail(hipError_t e, const char *what)
 {
@@ -351,7 +360,11 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     P->KI = ki_of(c.n_el);
     if (!P->KI) return fail(FN_ERR_UNSUPPORTED, "at most 128 array elements are supported (got %d)", c.n_el);
     P->MT_all = (c.n_tx + 15) / 16;
-    if (P->mode == 0) {
+    if (P->lbl) {
+        // line scan: one group per grid column, staging the M-tile of its transmit row
+        P->MT_tmpl = 1;
+        P->n_groups = c.n_x;
+    } else if (P->mode == 0) {
         // at most 5 transmit tiles per group keep the LDS tile within 80 KiB
         // (ensembles: 4, for the per-row image accumulators)
         const int max_tiles = P->nw ? 2 : tile_cap(P->n_frames, c.precision);
@@ -372,7 +385,7 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     if (P->lds_spec > kMaxLds) return fail(FN_ERR_UNSUPPORTED, "DFT order %d too large for the LDS twiddle table", g.N);
 
     // bin split: enough workgroups to fill 256 CUs several times over
-    const long tiles = (P->n_vox + 63) / 64;
+    const long tiles = beam_tiles(P);
     const long blocks = tiles * P->n_groups;
     int split = 1;  // windowed normalization: every block walks all bins (valid-row counts)
     if (blocks < 2048 && !P->nw) split = (int)std::min<long>((2048 + blocks - 1) / blocks, std::max(1, g.n_f / 8));
@@ -424,7 +437,7 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     A.n_f = g.n_f; A.n_lb = g.n_lb; A.N = g.N;
     A.bin_chunk = chunk;
     A.tx_group = P->MT_tmpl;
-    A.atomic_out = (P->bin_split > 1 || P->n_groups > 1) ? 1 : 0;
+    A.atomic_out = (P->bin_split > 1 || (P->n_groups > 1 && !P->lbl)) ? 1 : 0;  // line scan: a column per group
     A.M = (c.n_el - 1) / 2.0;
     A.pitch = c.pitch; A.width = c.width; A.c0 = c.c0; A.f_s = c.f_s; A.t0 = c.t0;
     // das_pw.m:384,467 (CPU: 2 * sum) vs gpu_bf_das_pw_rf.cu:1118-1119 (MEX: 2 / N * sum)
@@ -434,7 +447,7 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     S.n_t = c.n_t; S.n_ch = c.n_el * c.n_tx; S.N = g.N; S.n_lb = g.n_lb; S.n_f = g.n_f;
     S.n_el = c.n_el; S.n_tx = c.n_tx; S.MT = P->MT_all; S.KI = P->KI;
 
-    P->flops = 8.0 * g.n_f * (double)P->n_vox * c.n_el * c.n_tx;
+    P->flops = 8.0 * g.n_f * (double)P->n_vox * c.n_el * (P->lbl ? 1 : c.n_tx);
     return FN_OK;
 }
 
@@ -463,6 +476,7 @@ double useful_fraction(const fnh::Grid &g, const double *px, int nx, const doubl
 hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipStream_t s)
 {
     const int kidx = P->KI == 8 ? 0 : (P->KI == 16 ? 1 : 2);
+    if (P->lbl) return fnk::kLblLaunchers[kidx](P->wcr, grid, s, A);
     if (P->nw) return fnk::kNwLaunchers[P->mode][kidx](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
     if (P->n_frames > 1) return fnk::kEnsLaunchers[kidx][P->precision](P->MT_tmpl, P->wcr, grid, P->lds_beam, s, A);
     return fnk::kBeamLaunchers[P->mode][kidx][P->precision](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
@@ -572,7 +586,7 @@ int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w,
     if (P->profiling) HIP_CHECK(hipEventRecord(run.e0, s), "hipEventRecord");
     if (int st = prepare(P, d_rf, dtype, d_image, s)) return st;
     if (P->profiling) HIP_CHECK(hipEventRecord(run.e1, s), "hipEventRecord");
-    const long tiles = (P->n_vox + 63) / 64;
+    const long tiles = beam_tiles(P);
     hipError_t e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups), A, s);
     if (e != hipSuccess) return hip_fail(e, "beamform_kernel");
     if (P->profiling) {
@@ -607,6 +621,7 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
     if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
     P->mode = 0;
     P->n_frames = std::max(1, p->n_frames);
+    P->lbl = p->line_scan != 0;
     // windowed normalization: the CPU path tapers (windowed.m:81-86); the
     // reference MEX reads the window and normalizes as "on" (gpu_bf_das_pw_rf.cu:421-427)
     P->nw = p->normalization == FN_NORMALIZATION_WINDOWED && p->semantics == FN_SEMANTICS_CPU;
@@ -695,6 +710,7 @@ size_t dtype_size(int dtype) { return dtype == FN_DTYPE_F64 ? 8 : 4; }
 int max_tiles(const fn_plan *P)
 {
     if (P->mode == 1) return P->KI / 4;
+    if (P->lbl) return 1;
     return std::min(P->MT_all, P->nw ? 2 : tile_cap(P->n_frames, P->precision));
 }
 
@@ -715,12 +731,12 @@ int set_tiling(fn_plan *P, int mt, int split)
     if (lds > kMaxLds) return fail(FN_ERR_ARGUMENT, "transmit tile exceeds LDS");
     const int chunk = (n_f + split - 1) / split;
     P->MT_tmpl = mt;
-    P->n_groups = P->mode == 1 ? 1 : (P->MT_all + mt - 1) / mt;
+    P->n_groups = P->mode == 1 ? 1 : (P->lbl ? P->n_x : (P->MT_all + mt - 1) / mt);
     P->bin_split = (n_f + chunk - 1) / chunk;
     P->lds_beam = lds;
     P->args.tx_group = mt;
     P->args.bin_chunk = chunk;
-    P->args.atomic_out = (P->bin_split > 1 || P->n_groups > 1) ? 1 : 0;
+    P->args.atomic_out = (P->bin_split > 1 || (P->n_groups > 1 && !P->lbl)) ? 1 : 0;
     return FN_OK;
 }
 
@@ -806,6 +822,7 @@ int check_pw_fp32(const fn_plan *P, const char *what)
     if (P->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "%s: single-frame plans only", what);
     if (P->n_el > 128) return fail(FN_ERR_UNSUPPORTED, "%s: at most 128 elements", what);
     if (P->nw) return fail(FN_ERR_UNSUPPORTED, "%s: normalizations.windowed is not differentiated", what);
+    if (P->lbl) return fail(FN_ERR_UNSUPPORTED, "%s: line-scan plans are forward only", what);
     return FN_OK;
 }
 
@@ -958,6 +975,7 @@ int focused_signal(fn_plan *P, float *d_signal, hipStream_t s)
 {
     if (!P || !d_signal) return fail(FN_ERR_ARGUMENT, "null argument");
     if (P->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "focused signal: single-frame plans only");
+    if (P->lbl) return fail(FN_ERR_UNSUPPORTED, "focused signal: not available for line-scan plans");
     if (!P->executed) return fail(FN_ERR_ARGUMENT, "focused signal: execute the plan first");
     int vx = -1, vz = -1;  // last lateral position with a valid bin, its last such depth
     for (int ix = P->n_x - 1; ix >= 0 && vx < 0; --ix)
@@ -1242,7 +1260,7 @@ static void batch_release_graph(fn_batch *b)
 // (ensembles, windowed normalization)
 static long batch_class_key(const fn_plan *P)
 {
-    if (P->n_frames > 1 || P->nw) return -1;
+    if (P->n_frames > 1 || P->nw || P->lbl) return -1;
     return ((((long)P->mode * 64 + P->KI) * 16 + P->MT_tmpl) * 8 + P->wcr) * 8 * 4 + P->wct * 4 + P->precision;
 }
 
@@ -1384,7 +1402,7 @@ int32_t fn_batch_execute(fn_batch *b, const void *const *d_rf, const int32_t *dt
             fn_plan *P = b->plans[i];
             hipStream_t ws = b->streams[slot++ % W];
             st = prepare(P, d_rf[i], dtypes[i], d_images[i], ws);
-            const long tiles = (P->n_vox + 63) / 64;
+            const long tiles = beam_tiles(P);
             if (!st)
                 e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups),
                                 beam_args(P, d_images[i], nullptr, nullptr), ws);

@@ -314,6 +314,16 @@ int validate_pw(const fn_pw_params *p, Grid *g)
     if (p->semantics == FN_SEMANTICS_CPU && p->window.id == FN_WINDOW_COSINE)
         return fail(FN_ERR_WINDOW, "the cosine window has no MATLAB class (CPU semantics)");
     if ((s = check_norm(p->normalization, p->normalization_window))) return s;
+    if (p->line_scan) {
+        if (p->line_scan != 1) return fail(FN_ERR_ARGUMENT, "line_scan must be 0 or 1 (got %d)", p->line_scan);
+        if (p->n_angles != p->n_x)
+            return fail(FN_ERR_ARGUMENT, "line scan: one transmit per grid column (n_angles %d, n_x %d)", p->n_angles, p->n_x);
+        if (p->n_x > 65535) return fail(FN_ERR_UNSUPPORTED, "line scan: at most 65535 lines per plan");
+        if (p->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "line scan: one frame per plan");
+        if (p->precision != FN_PRECISION_FP32) return fail(FN_ERR_UNSUPPORTED, "line scan: FP32 only");
+        if (p->normalization == FN_NORMALIZATION_WINDOWED && p->semantics == FN_SEMANTICS_CPU)
+            return fail(FN_ERR_UNSUPPORTED, "line scan: normalizations.windowed (CPU semantics) is not supported");
+    }
     Grid gg;
     if ((s = frequency_grid(p->n_samples, p->n_elements, p->element_pitch, z_pad_of(p->positions_z, p->n_z, p->semantics),
                             p->f_s, p->c_0, p->f_lb, p->f_ub, &gg)))

@@ -523,24 +523,27 @@ struct PersistWavesPerEU {
 // transmit group bz.  beamform_kernel runs it for its block index; the
 // persistent batch kernel for every work item it claims.
 template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false,
-          typename Args = BeamArgs>
+          bool LBL = false, typename Args = BeamArgs>
 __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by, const int bz, const int tid)
 {
     static_assert(!TAN || (MODE == MODE_PW && PREC == PREC_FP32), "tangent: plane waves, FP32");
     static_assert(!ENS || (MODE == MODE_PW && !TAN), "ensemble: plane waves");
     static_assert(!NW || (PREC == PREC_FP32 && !TAN && !ENS), "windowed normalization: FP32 forward");
+    static_assert(!LBL || (MODE == MODE_PW && MT == 1 && !TAN && !ENS && !NW), "line scan: plane-wave forward, one tile");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, c = lane & 15;
     const long n_vox = (long)A.n_x * A.n_z;
-    const long v = (long)bx * 64 + wave * 16 + c;
-    const bool vok = v < n_vox;
-    const long vv = vok ? v : n_vox - 1;
+    // line scan: block bz covers depths of grid column bz, the line of transmit row bz
+    const long v_loc = (long)bx * 64 + wave * 16 + c;
+    const long v = LBL ? (long)bz * A.n_z + v_loc : v_loc;
+    const bool vok = LBL ? v_loc < A.n_z : v < n_vox;
+    const long vv = vok ? v : (LBL ? (long)bz * A.n_z + A.n_z - 1 : n_vox - 1);
     const int ix = (int)(vv / A.n_z), iz = (int)(vv - (long)ix * A.n_z);
     const double x = A.px[ix], z = A.pz[iz];
     const double xp = x / A.pitch, zp = z / (2.0 * A.pitch);
     const double clamp_z = sqrt(z / (2.88 * A.c0));  // sqrt(f z / (2.88 c)) = sqrt(f) clamp_z (das_pw.m:374)
-    const int mt0 = bz * A.tx_group;  // first M-tile of this transmit group
+    const int mt0 = LBL ? (bz >> 4) : bz * A.tx_group;  // first M-tile of this transmit group
 
     // receive phase increments: tau_rx = sqrt(((x_k-x)/c)^2 + (z/c)^2) (das_pw.m:339-340,427)
     // plus the element's delay offset
@@ -933,7 +936,9 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     for (int r = 0; r < 4; ++r) {
                         const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
                         const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                        const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
+                        // line scan: only the row of this block's line contributes
+                        const float scr = (LBL && 16 * (mt0 + mt) + 4 * g + r != bz) ? 0.f : sc;
+                        const float yr = accR[mt][r] * scr, yi = accI[mt][r] * scr;
                         if (ENS) {
                             Er[ENS ? mt : 0][r] += cs * yr - sn * yi;
                             Ei[ENS ? mt : 0][r] += cs * yi + sn * yr;
@@ -1062,14 +1067,15 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             A.image[v] = val;
             if (TAN) dimg[v] = make_float2(A.out_scale * Dr, A.out_scale * Di);
         }
-        if (b_end == A.n_f && bz == 0) {
+        if (b_end == A.n_f && (LBL || bz == 0)) {
             if (A.weights) A.weights[v] = w_last;
             if (MODE == MODE_SA && A.weights_tx) A.weights_tx[v] = wtx_last;
         }
     }
 }
 
-template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false>
+template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false,
+          bool LBL = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BeamWavesPerEU<MT, KI, WCR, PREC, ENS>::value)))
 void beamform_kernel(BeamArgs A)
 {
@@ -1078,7 +1084,8 @@ void beamform_kernel(BeamArgs A)
     // reference form leaves the compiler the registers it schedules best with
     using Args =
         typename std::conditional<(MT * KI <= 64 || PREC != PREC_FP32), BeamArgs, const BeamArgs &>::type;
-    beamform_tile<MODE, MT, KI, WCR, WCT, PREC, TAN, ENS, NW, Args>(A, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x);
+    beamform_tile<MODE, MT, KI, WCR, WCT, PREC, TAN, ENS, NW, LBL, Args>(A, blockIdx.x, blockIdx.y, blockIdx.z,
+                                                                      threadIdx.x);
 }
 
 // Persistent batch form: the work items (job, voxel tile, bin chunk, transmit
@@ -1104,7 +1111,7 @@ void beamform_persistent_kernel(const BeamArgs *__restrict__ jobs, const int4 *_
         // recomputed per tile rather than hoisted out of the loop and held across tiles
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        beamform_tile<MODE, MT, KI, WCR, WCT, PREC, false, false, false, const BeamArgs &>(
+        beamform_tile<MODE, MT, KI, WCR, WCT, PREC, false, false, false, false, const BeamArgs &>(
             A, __builtin_amdgcn_readfirstlane(w.y),
                                                     __builtin_amdgcn_readfirstlane(w.z),
                                                     __builtin_amdgcn_readfirstlane(w.w), tid);

@@ -123,9 +123,10 @@ def _image_out(buf, n_x, n_z):
 
 def pw_params(px, pz, n_t, n_el, angles, f_s, w, p, c, fb, t0, window, F, normalization, index_gpu=0,
               semantics="cpu", precision="fp32", output_scale="mex", verbosity=0, n_frames=1,
-              virtual_sources=None):
+              virtual_sources=None, line_scan=False):
     """Build the fn_pw_params struct (and the arrays it points to) for libfnumber_hip.
-    virtual_sources: (N_a, 2) array of (x_vs, z_vs) in m replacing the steering angles."""
+    virtual_sources: (N_a, 2) array of (x_vs, z_vs) in m replacing the steering angles.
+    line_scan: grid column i is formed from transmit i alone (N_a == N_x)."""
     L = _lib.load()
     px = np.ascontiguousarray(px, dtype=np.float64)
     pz = np.ascontiguousarray(pz, dtype=np.float64)
@@ -148,6 +149,7 @@ def pw_params(px, pz, n_t, n_el, angles, f_s, w, p, c, fb, t0, window, F, normal
     P.index_gpu = int(index_gpu)
     P.verbosity = int(verbosity)
     P.n_frames = int(n_frames)
+    P.line_scan = int(bool(line_scan))
     vs = None
     if virtual_sources is not None:
         vs = np.ascontiguousarray(np.asarray(virtual_sources, dtype=np.float64).reshape(-1, 2))

@@ -143,11 +143,13 @@ class PwPlan(_Plan):
     def __init__(self, positions_x, positions_z, n_samples, n_elements, steering_angles, f_s, element_width,
                  element_pitch, c_0, f_bounds, index_t0=0.0, window=None, F_number=None, normalization=None,
                  index_gpu=0, semantics="cpu", precision="fp32", output_scale="mex", n_frames=1,
-                 virtual_sources=None):
+                 virtual_sources=None, line_scan=False):
         """n_frames > 1: an ensemble of frames with this geometry -- RF (n_frames, N_a, N_el, N_t),
         images (n_frames, N_x, N_z), each frame compounding its own angles.
         virtual_sources: (N_a, 2) (x_vs, z_vs) in m per transmit instead of plane waves
-        (z_vs < 0 diverging, z_vs > 0 focused; steering_angles then only gives N_a)."""
+        (z_vs < 0 diverging, z_vs > 0 focused; steering_angles then only gives N_a).
+        line_scan: column i of the grid is formed from transmit i alone (N_a == N_x;
+        fnumber_hip.h fn_pw_params.line_scan), FP32 forward only."""
         super().__init__()
         window = window if window is not None else windows.tukey(0.2)
         F_number = F_number if F_number is not None else f_numbers.grating.angle_lb(60, 1.5)
@@ -155,7 +157,7 @@ class PwPlan(_Plan):
         P, keep, info = pw_params(positions_x, positions_z, n_samples, n_elements, steering_angles, f_s,
                                   element_width, element_pitch, c_0, f_bounds, index_t0, window, F_number,
                                   normalization, index_gpu, semantics, precision, output_scale, n_frames=n_frames,
-                                  virtual_sources=virtual_sources)
+                                  virtual_sources=virtual_sources, line_scan=line_scan)
         _lib.check(_lib.load().fn_plan_pw_create(C.byref(P), C.byref(self._h)), self.prefix)
         self._keep = keep
         self.grid = info

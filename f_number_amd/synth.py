@@ -165,6 +165,16 @@ def diverging_waves(n_sources: int = 11, z_vs: float = -15e-3) -> Workload:
     return w
 
 
+def line_scan(n_lines: int = 128, n_z: int = 256, focus_z: float = 20e-3) -> Workload:
+    """Line-by-line scan: transmit i focused at (x_i, focus_z), x_i = positions_x[i] spread
+    over [-19, 19] mm; 2048 samples, 2.25-6.75 MHz, n_z depths over z in [5, 40] mm.  RF
+    (N_lines, N_el, N_t) holds one transmit event per image line."""
+    w = Workload(f"line_scan_{n_lines}x{n_z}", "pw", 2048, np.zeros(n_lines), _lin(-19e-3, 19e-3, n_lines),
+                 _lin(5e-3, 40e-3, n_z), (2.25e6, 6.75e6), seed=20260006)
+    w.virtual_sources = np.stack([w.positions_x, np.full(n_lines, focus_z)], axis=1)
+    return w
+
+
 def autofocus_layer() -> Workload:
     """Differentiation-form acquisition: the 11-angle array of cpwc_small (2048 samples,
     2.25-6.75 MHz) over a 6 mm surface layer at 1450 m/s on a 1540 m/s background, with

@@ -148,6 +148,13 @@ typedef struct fn_pw_params {
      * the array centre); z_vs > 0: focused wave, (|r_vs| +- |r - r_vs|) / c
      * (+ below the focus).  Both plus index_t0 / f_s. */
     const double *virtual_sources;
+    /* line-by-line scan (README.md:31-33 names the application; the reference
+     * has no code for it): 0 off; 1: grid column i is formed from transmit i
+     * alone (n_x == n_angles, e.g. one focused virtual source per line), so an
+     * image of n_x lines is one plan, one spectra launch and one beamforming
+     * launch.  FP32, one frame, forward only: the tangent, adjoint, vjp and
+     * focused-signal calls and windowed normalization (CPU semantics) refuse it. */
+    int32_t line_scan;
 } fn_pw_params;
 
 /* Full synthetic aperture (das_sa.m:1, gpu_bf_saft_rf.cu:107).

@@ -1,0 +1,145 @@
+"""Line-by-line scanning (f_number_amd.linescan) on the GPU.
+
+Line i of the scan is das_pw on the single grid column x_i with one transmit
+focused at (x_i, z_f): each column is checked against the oracle's das_pw with
+that virtual source (rel-L2 <= 1e-4, peak <= -80 dB, output_scale='cpu'), and a
+synthesized focused-wave scan of point scatterers images them at their positions.
+"""
+import numpy as np
+import pytest
+
+from conftest import peak_db, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_L2 = 1e-4
+TOL_DB = -80.0
+
+
+@pytest.fixture(scope="module")
+def fa():
+    import f_number_amd as fa
+
+    fa.load()
+    return fa
+
+
+@pytest.mark.parametrize("win,sem,focus_x", [("tukey", "cpu", "lines"), ("hann", "mex", "lines"),
+                                             ("boxcar", "cpu", "steered")])
+def test_line_scan_matches_oracle(fa, oracle, win, sem, focus_x):
+    from f_number_amd import synth
+
+    n_lines, n_z, n_el, n_t = 12, 40, 64, 1024
+    w = synth.line_scan(n_lines, n_z)
+    x = np.linspace(-8e-3, 8e-3, n_lines)
+    fx = x if focus_x == "lines" else x + 1.5e-3      # focus off the line (steered transmit)
+    z, fz = w.positions_z, 18e-3
+    rng = np.random.default_rng(47)
+    rf = rng.standard_normal((n_lines, n_el, n_t)).astype(np.float32)
+    window = {"tukey": fa.windows.tukey(0.5), "hann": fa.windows.hann(), "boxcar": fa.windows.boxcar()}[win]
+    Fo = fa.f_numbers.grating.angle_lb(60, 1.5)
+    print(f"case line scan win={win} sem={sem} focus={focus_x} lines={n_lines}", flush=True)
+    img = fa.das_lbl(x, z, np.transpose(rf, (2, 1, 0)), w.f_s, fz, w.width, w.pitch, w.c_0, w.f_bounds, 0.0,
+                     window, Fo, fa.normalizations.on(), fa.platforms.gpu(0), focus_x=fx, semantics=sem,
+                     output_scale="cpu")
+    assert img.shape == (n_z, n_lines) and img.dtype == np.complex64
+    F_o = oracle.fnumber("angle_lb", Fo.angle_lb_deg, Fo.F_number_ub, Fo.distance_deg)
+    ref = np.empty((n_lines, n_z), dtype=np.complex128)
+    for i in range(n_lines):
+        col, _ = oracle.das_pw(x[i:i + 1], z, rf[i:i + 1], w.f_s, np.zeros(1), w.width, w.pitch, w.c_0, w.f_bounds,
+                               0.0, (win, 0.5 if win == "tukey" else 0.0), F_o, "on", semantics=sem,
+                               virtual_sources=[[fx[i], fz]])
+        ref[i] = np.asarray(col).reshape(-1)
+    e, d = rel_l2(img.T, ref), peak_db(img.T, ref)
+    print(f"  rel_l2={e:.3e} peak={d:.1f} dB", flush=True)
+    assert e <= TOL_L2 and d <= TOL_DB
+
+
+def test_line_scan_matches_single_line_plans(fa):
+    """The line-scan plan equals one single-column plan per line (each with its one
+    virtual source, run as a Batch), and a second execute on new data in the same
+    buffers equals a fresh scan of that data."""
+    import torch
+    from f_number_amd import synth
+
+    w = synth.line_scan(40, 64)
+    w.n_samples, w.n_speckle = 1024, 500
+    rf = w.rf_device("cuda:0")
+    w2 = synth.line_scan(40, 64)
+    w2.n_samples, w2.n_speckle, w2.seed = 1024, 500, w.seed + 7
+    rf2 = w2.rf_device("cuda:0")
+    print("case line scan vs single-line plans, 40 lines", flush=True)
+    scan = fa.LineScan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.f_s, w.width, w.pitch, w.c_0,
+                       w.f_bounds, 20e-3)
+    img = torch.empty((40, 64), dtype=torch.complex64, device="cuda")
+    scan.execute(rf, img)
+    plans = [fa.PwPlan(w.positions_x[i:i + 1], w.positions_z, w.n_samples, w.n_elements, [0.0], w.f_s, w.width,
+                       w.pitch, w.c_0, w.f_bounds, virtual_sources=[[w.positions_x[i], 20e-3]]) for i in range(40)]
+    ref = torch.empty((40, 64), dtype=torch.complex64, device="cuda")
+    batch = fa.Batch(plans)
+    batch.execute([rf[i] for i in range(40)], [ref[i] for i in range(40)])
+    torch.cuda.synchronize()
+    e = rel_l2(img.cpu().numpy(), ref.cpu().numpy())
+    print(f"  rel_l2 vs single-line plans {e:.3e}", flush=True)
+    assert e <= 1e-6
+    batch.close()
+    for p in plans:
+        p.close()
+    rf.copy_(rf2)
+    scan.execute(rf, img)
+    torch.cuda.synchronize()
+    got = img.cpu().numpy()
+    scan.close()
+    fresh = fa.das_lbl(w.positions_x, w.positions_z, np.transpose(rf2.cpu().numpy(), (2, 1, 0)), w.f_s, 20e-3,
+                       w.width, w.pitch, w.c_0, w.f_bounds)
+    assert rel_l2(got, fresh.T) <= 1e-6
+
+
+def test_line_scan_plan_is_forward_only(fa):
+    import torch
+    from f_number_amd import synth
+
+    w = synth.line_scan(8, 32)
+    print("case line scan refusals", flush=True)
+    scan = fa.LineScan(w.positions_x, w.positions_z, 512, 32, w.f_s, w.width, w.pitch, w.c_0, w.f_bounds, 20e-3)
+    y = torch.zeros((8, 32), dtype=torch.complex64, device="cuda")
+    with pytest.raises(fa.FNumberError, match="forward only"):
+        scan.plan.adjoint(y)
+    with pytest.raises(fa.FNumberError, match="forward only"):
+        scan.plan.execute_tangent(torch.zeros((8, 32, 512), device="cuda"), y, y.clone())
+    scan.close()
+    with pytest.raises(fa.FNumberError, match="one transmit per grid column"):
+        fa.PwPlan(w.positions_x, w.positions_z, 512, 32, np.zeros(7), w.f_s, w.width, w.pitch, w.c_0, w.f_bounds,
+                  virtual_sources=w.virtual_sources[:7], line_scan=True)
+    with pytest.raises(fa.FNumberError, match="FP32 only"):
+        fa.PwPlan(w.positions_x, w.positions_z, 512, 32, np.zeros(8), w.f_s, w.width, w.pitch, w.c_0, w.f_bounds,
+                  precision="fp16", virtual_sources=w.virtual_sources, line_scan=True)
+
+
+def test_line_scan_focuses_points(fa):
+    """Synthesized focused-wave scan (transmit i focused at (x_i, 20 mm)): each point
+    scatterer's envelope peaks within one line and one depth sample of its position."""
+    import torch
+    from f_number_amd import synth
+
+    w = synth.line_scan(96, 256)
+    w.n_points, w.n_speckle, w.snr_db = 6, 0, np.inf
+    xs, zs, _ = w.scatterers()
+    print("case line scan points 96 lines", flush=True)
+    scan = fa.LineScan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.f_s, w.width, w.pitch, w.c_0,
+                       w.f_bounds, 20e-3, window=fa.windows.hann())
+    img = torch.empty((96, 256), dtype=torch.complex64, device="cuda")
+    scan.execute(w.rf_device("cuda:0"), img)
+    a = img.abs().cpu().numpy()
+    scan.close()
+    dx, dz = np.diff(w.positions_x[:2])[0], np.diff(w.positions_z[:2])[0]
+    checked = 0
+    for xp, zp in zip(xs, zs):
+        ix, iz = int(round((xp - w.positions_x[0]) / dx)), int(round((zp - w.positions_z[0]) / dz))
+        if not (3 <= ix < a.shape[0] - 3 and 3 <= iz < a.shape[1] - 3):
+            continue
+        win = a[ix - 3:ix + 4, iz - 3:iz + 4]
+        jx, jz = np.unravel_index(np.argmax(win), win.shape)
+        assert abs(jx - 3) <= 1 and abs(jz - 3) <= 1, (xp, zp)
+        checked += 1
+    assert checked >= 3

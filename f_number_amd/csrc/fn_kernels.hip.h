@@ -583,6 +583,8 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             Qtx[mt][r] = frac32(A.f_s * tau / A.N);
             if (TAN) Ttx[TAN ? mt : 0][r] = (float)tg;
         }
+    // line scan: the phase increment of the block's own transmit row
+    const uint32_t Qln = LBL ? frac32(A.f_s * (pw_tx_time(A, bz, x, z) + A.t0 / A.f_s) / A.N) : 0u;
     const int k0 = kelem<PREC>(0, g);
     const float dg = (float)(((double)k0 - A.M) * A.pitch - x);  // x_k - x of element j = 0
     const float pf = (float)A.pitch;
@@ -763,6 +765,11 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             //   Re Y = K1 - K3, Im Y = K1 + K2
             floatx4 K1[MT], K2[MT], K3[MT];
             floatx4 T1[TAN ? MT : 1], T2[TAN ? MT : 1], T3[TAN ? MT : 1];
+            // line scan: one transmit row contributes, so the product runs on the vector ALUs
+            // (the lane's elements 4 j + g against row bz of the staged tile, read as a
+            // broadcast) instead of a 16-row matrix-core product of which one row is used
+            float Lr = 0.f, Li = 0.f;
+            const int lrow = bz & 15;
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 K1[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -808,6 +815,14 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 }
                 const float br = a * cs, bi = a * sn;
                 const float bd = bi - br, bp = br + bi;
+                if constexpr (LBL) {
+                    const float2 xv = xs[(j - cc * SJC) * 64 + lrow + 16 * g];
+                    Lr = fmaf(xv.x, br, Lr);
+                    Lr = fmaf(-xv.y, bi, Lr);
+                    Li = fmaf(xv.x, bi, Li);
+                    Li = fmaf(xv.y, br, Li);
+                    continue;
+                }
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
                     const float2 xv = xs[(mt * SJC + (j - cc * SJC)) * 64 + lane];
@@ -829,6 +844,10 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             chunk_step(std::integral_constant<int, 1>{});
             chunk_step(std::integral_constant<int, 2>{});
             chunk_step(std::integral_constant<int, 3>{});
+            if constexpr (LBL) {
+                accR[0] = floatx4{Lr, 0.f, 0.f, 0.f};
+                accI[0] = floatx4{Li, 0.f, 0.f, 0.f};
+            } else
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 accR[mt] = K1[mt] - K3[mt];
@@ -930,15 +949,21 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 const float sc = (A.norm_rx ? 1.0f / asum : 1.0f) * xscale;
                 // padded transmit rows have zero spectra, so every row is summed
                 float pr = 0.f, pi = 0.f;  // tangent: sum_rows T (tau_tx,g Y + Y')
+                if constexpr (LBL) {
+                    // the lane's partial product over its elements, at the line's own transmit phase
+                    const float rv = rev_of((uint32_t)n * Qln);
+                    const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
+                    const float yr = accR[0][0] * sc, yi = accI[0][0] * sc;
+                    Ir += cs * yr - sn * yi;
+                    Ii += cs * yi + sn * yr;
+                } else
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
                         const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                        // line scan: only the row of this block's line contributes
-                        const float scr = (LBL && 16 * (mt0 + mt) + 4 * g + r != bz) ? 0.f : sc;
-                        const float yr = accR[mt][r] * scr, yi = accI[mt][r] * scr;
+                        const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
                         if (ENS) {
                             Er[ENS ? mt : 0][r] += cs * yr - sn * yi;
                             Ei[ENS ? mt : 0][r] += cs * yi + sn * yr;

@@ -320,17 +320,31 @@ __global__ __launch_bounds__(256) void spectra_kernel(const T *__restrict__ rf, 
         accR[j] = floatx4{0.f, 0.f, 0.f, 0.f};
         accI[j] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
+    // RF chunks of 64 samples x 64 channels: thread tid holds samples tid & 63 of channels
+    // (tid >> 6) + 4 i, i = 0..15; the next chunk is loaded into registers while the
+    // current one is contracted
+    // (channel cl = (tid >> 6) + 4 i is row 16 A + (cl & 15), element 4 K4 + (cl >> 4), so the
+    // per-i offsets from the thread's base are wave-uniform)
+    const int tl_own = tid & 63, cl_own = tid >> 6;
+    const int a_own = 16 * A + cl_own;
+    const T *src = rf + ((size_t)a_own * S.n_el + 4 * K4) * S.n_t + tl_own;
+    float pre[16];
+    auto load_chunk = [&](int t0) {
+        const bool tok = t0 + tl_own < S.n_t;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int ra = 4 * (i & 3), ke = i >> 2;
+            const bool ok = tok && a_own + ra < S.n_tx && 4 * K4 + ke < S.n_el;
+            pre[i] = ok ? (float)src[((size_t)ra * S.n_el + ke) * S.n_t + t0] : 0.f;
+        }
+    };
+    load_chunk(0);
     for (int t0 = 0; t0 < S.n_t; t0 += 64) {
         __syncthreads();
-        for (int e = tid; e < 64 * 64; e += 256) {
-            const int cl = e >> 6, tl = e & 63;          // channel of the group, sample of the chunk
-            const int a = 16 * A + (cl & 15), k = 4 * K4 + (cl >> 4);
-            const int t = t0 + tl;
-            float v = 0.f;
-            if (a < S.n_tx && k < S.n_el && t < S.n_t) v = (float)rf[((size_t)a * S.n_el + k) * S.n_t + t];
-            xs[cl * 65 + tl] = v;
-        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) xs[(cl_own + 4 * i) * 65 + tl_own] = pre[i];
         __syncthreads();
+        if (t0 + 64 < S.n_t) load_chunk(t0 + 64);
         int m = (int)(((int64_t)n * (t0 + q)) % S.N);
         const int n_steps = (min(64, S.n_t - t0) + 3) >> 2;  // samples past n_t are staged as zeros
         for (int st = 0; st < n_steps; ++st) {

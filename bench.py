@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form measurements")
     ap.add_argument("--no-batching", action="store_true", help="skip the batching-form measurements")
     ap.add_argument("--no-saft", action="store_true", help="skip the SAFT precision-form measurements")
+    ap.add_argument("--no-line-scan", action="store_true", help="skip the line-by-line scanning measurement")
     ap.add_argument("--ensemble-frames", type=int, default=400)
     ap.add_argument("--queue-jobs", type=int, default=64)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -304,6 +305,43 @@ def run_diff(steps, warmup, dev):
     }
 
 
+def run_line_scan(steps, warmup, dev):
+    """Line-by-line scan: 128 lines over x in [-19, 19] mm, each from its own transmit focused
+    at (x_i, 20 mm), 2048 samples, 512 depths over z in [5, 40] mm (one line-scan plan: one
+    spectra launch over the 128 transmit events, one beamforming launch)."""
+    import torch
+
+    import f_number_amd as fa
+    from f_number_amd import synth
+
+    w = synth.line_scan(128, 512)
+    rf = w.rf_device(dev)
+    scan = fa.LineScan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.f_s, w.width, w.pitch, w.c_0,
+                       w.f_bounds, 20e-3, index_gpu=dev.index or 0)
+    img = torch.empty((128, 512), dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    t_frame = _time_ms(lambda: scan.execute(rf, img, stream=stream), steps, warmup, dev)
+    scan.plan.set_profiling(True)
+    scan.plan.get_profile()
+    for _ in range(steps):
+        scan.execute(rf, img, stream=stream)
+    torch.cuda.synchronize(dev)
+    spec_ms, beam_ms, n = scan.plan.get_profile()
+    scan.plan.set_profiling(False)
+    info = scan.info()
+    scan.close()
+    triples = float(info["n_voxels"]) * info["n_f"] * w.n_elements
+    beam = beam_ms / max(n, 1)
+    return {
+        "workload": w.name, "n_lines": 128, "n_voxels": int(w.n_voxels), "frame_ms": t_frame,
+        "frames_per_s": 1e3 / t_frame, "mvox_per_s": w.n_voxels / t_frame / 1e3,
+        "stages_ms": {"spectra": spec_ms / max(n, 1), "beamform": beam},
+        "beamform_weights_per_s": triples / (beam / 1e3),
+        "note": "beamforming bound by the vector ALUs: one weight (window, phase sine and cosine) per "
+                "(voxel, bin, element), used by one transmit row",
+    }
+
+
 def run_batching(steps, n_frames, n_jobs, dev):
     """Batching form: a Doppler ensemble (n_frames x 5 angles, 1536 samples, 256 x 256,
     flow phantom) and a queue of n_jobs independent CPWC jobs (grids and angle counts drawn
@@ -512,6 +550,8 @@ def main():
     }
     if not args.no_diff:
         out["differentiation"] = run_diff(max(3, args.steps // 2), 1, dev)
+    if not args.no_line_scan:
+        out["line_scan"] = run_line_scan(max(3, args.steps), 2, dev)
     if not args.no_batching:
         out["batching"] = run_batching(max(2, args.steps // 3), args.ensemble_frames, args.queue_jobs, dev)
     saft = None

@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from oracle import oracle as O  # noqa: E402
 
-# array and acquisition of the reference's configurations (SURVEY cfg0 geometry)
+# array and acquisition: 128-element linear array, 0.3048 mm pitch, 0.27 mm width, 20.832 MHz, 1540 m/s
 GEO = dict(f_s=20.832e6, width=0.27e-3, pitch=0.3048e-3, c_0=1540.0, f_bounds=(2.25e6, 6.75e6))
 
 F_KINDS = {

@@ -388,7 +388,13 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     const long tiles = beam_tiles(P);
     const long blocks = tiles * P->n_groups;
     int split = 1;  // windowed normalization: every block walks all bins (valid-row counts)
-    if (blocks < 2048 && !P->nw) split = (int)std::min<long>((2048 + blocks - 1) / blocks, std::max(1, g.n_f / 8));
+    if (P->lbl) {
+        // line scans (vector-ALU bound, one column per group): about 8192 blocks below
+        // 4096, measured best over bin splits 1..64 at 64-256 lines x 256-1024 depths
+        if (blocks < 4096) split = (int)std::min<long>({(8192 + blocks - 1) / blocks, 16L, (long)std::max(1, g.n_f / 8)});
+    } else if (blocks < 2048 && !P->nw) {
+        split = (int)std::min<long>((2048 + blocks - 1) / blocks, std::max(1, g.n_f / 8));
+    }
     const int chunk = (g.n_f + split - 1) / split;
     P->bin_split = (g.n_f + chunk - 1) / chunk;
 

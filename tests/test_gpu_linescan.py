@@ -143,3 +143,38 @@ def test_line_scan_focuses_points(fa):
         assert abs(jx - 3) <= 1 and abs(jz - 3) <= 1, (xp, zp)
         checked += 1
     assert checked >= 3
+
+
+def test_line_scan_plan_in_a_persistent_batch(fa):
+    """A line-scan plan runs on its own inside a persistent batch (one launch over its
+    lines) beside plane-wave plans, and matches its eager execution."""
+    import torch
+    from f_number_amd import synth
+
+    w = synth.line_scan(24, 48)
+    w.n_samples, w.n_speckle = 1024, 300
+    rf = w.rf_device("cuda:0")
+    scan = fa.LineScan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.f_s, w.width, w.pitch, w.c_0,
+                       w.f_bounds, 20e-3)
+    c = synth.scaled(synth.cpwc_small(), 32, 24, n_angles=3, n_samples=1024, n_speckle=300)
+    rfc = c.rf_device("cuda:0")
+    pws = [fa.PwPlan(c.positions_x, c.positions_z, c.n_samples, c.n_elements, c.angles, c.f_s, c.width, c.pitch,
+                     c.c_0, c.f_bounds) for _ in range(2)]
+    print("case line scan in a persistent batch", flush=True)
+    eager = torch.empty((24, 48), dtype=torch.complex64, device="cuda")
+    scan.execute(rf, eager)
+    eager_pw = torch.empty((32, 24), dtype=torch.complex64, device="cuda")
+    pws[0].execute(rfc, eager_pw)
+    imgs = [torch.empty((24, 48), dtype=torch.complex64, device="cuda")] + \
+        [torch.empty((32, 24), dtype=torch.complex64, device="cuda") for _ in range(2)]
+    batch = fa.Batch([scan.plan] + pws, persistent=True)
+    for _ in range(2):
+        batch.execute([rf, rfc, rfc], imgs)
+    torch.cuda.synchronize()
+    assert rel_l2(imgs[0].cpu().numpy(), eager.cpu().numpy()) <= 1e-6
+    for im in imgs[1:]:
+        assert rel_l2(im.cpu().numpy(), eager_pw.cpu().numpy()) <= 1e-6
+    batch.close()
+    scan.close()
+    for p in pws:
+        p.close()

@@ -24,14 +24,19 @@ def fa():
     return fa
 
 
-@pytest.mark.parametrize("win,sem,focus_x", [("tukey", "cpu", "lines"), ("hann", "mex", "lines"),
-                                             ("boxcar", "cpu", "steered")])
-def test_line_scan_matches_oracle(fa, oracle, win, sem, focus_x):
+@pytest.mark.parametrize("win,sem,focus_x,n_lines,n_z", [("tukey", "cpu", "lines", 12, 40),
+                                                         ("hann", "mex", "lines", 12, 40),
+                                                         ("boxcar", "cpu", "steered", 12, 40),
+                                                         ("tukey", "cpu", "lines", 1, 37),
+                                                         ("hann", "cpu", "steered", 17, 65)])
+def test_line_scan_matches_oracle(fa, oracle, win, sem, focus_x, n_lines, n_z):
+    """Single line, ragged depth counts (37, 65: partial 64-depth tiles) and 17 lines
+    (a second, partly filled 16-row transmit tile) included."""
     from f_number_amd import synth
 
-    n_lines, n_z, n_el, n_t = 12, 40, 64, 1024
+    n_el, n_t = 64, 1024
     w = synth.line_scan(n_lines, n_z)
-    x = np.linspace(-8e-3, 8e-3, n_lines)
+    x = np.linspace(-8e-3, 8e-3, n_lines) if n_lines > 1 else np.array([1.3e-3])
     fx = x if focus_x == "lines" else x + 1.5e-3      # focus off the line (steered transmit)
     z, fz = w.positions_z, 18e-3
     rng = np.random.default_rng(47)

@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import f_numbers, normalizations, windows
+from . import f_numbers, normalizations, platforms, windows
 from ._lib import FNumberError
 from .plan import PwPlan
 
@@ -90,8 +90,6 @@ def das_lbl(positions_x, positions_z, data_RF, f_s, focus_z, element_width, elem
     complex64 in the das_pw output scale."""
     import torch
 
-    from .platforms import gpu
-
     rf = np.asarray(data_RF)
     if rf.ndim == 2:
         rf = rf[:, :, None]
@@ -102,8 +100,12 @@ def das_lbl(positions_x, positions_z, data_RF, f_s, focus_z, element_width, elem
         _err("InvalidRFData", "one transmit event (third dimension of data_RF) per entry of positions_x")
     if f_bounds is None or np.size(f_bounds) == 0:
         f_bounds = (np.finfo(float).eps, float(f_s) / 2)  # the das_pw default
-    plat = platform if platform is not None else gpu(0)
-    index_gpu = getattr(plat, "index", 0)
+    plat = platform if platform is not None else platforms.gpu(0)
+    if not isinstance(plat, platforms.platform):
+        _err("NoPlatform", "platform must be scalar platforms.platform!")
+    if not isinstance(plat, platforms.gpu):
+        _err("UnsupportedPlatform", "only platforms.gpu is served")
+    index_gpu = plat.index
     scan = LineScan(positions_x, positions_z, n_t, n_el, f_s, element_width, element_pitch, c_0, f_bounds, focus_z,
                     focus_x, index_t0, window, F_number, normalization, index_gpu, semantics, output_scale)
     try:

@@ -87,3 +87,13 @@ def test_line_scan_flag_is_zero_or_one():
     P.line_scan = 2
     got, text = _validate(P)
     assert got == "InvalidArgument" and "line_scan" in text
+
+
+def test_line_scan_platform_must_be_a_gpu():
+    w = synth.line_scan(8, 16)
+    rf = np.zeros((w.n_samples, w.n_elements, 8), dtype=np.float32)
+    with pytest.raises(fa.FNumberError, match="das_lbl:UnsupportedPlatform"):
+        fa.das_lbl(w.positions_x, w.positions_z, rf, w.f_s, 20e-3, w.width, w.pitch, w.c_0,
+                   platform=fa.platforms.cpu())
+    with pytest.raises(fa.FNumberError, match="das_lbl:NoPlatform"):
+        fa.das_lbl(w.positions_x, w.positions_z, rf, w.f_s, 20e-3, w.width, w.pitch, w.c_0, platform="gpu")

@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-batching", action="store_true", help="skip the batching-form measurements")
     ap.add_argument("--no-saft", action="store_true", help="skip the SAFT precision-form measurements")
     ap.add_argument("--no-line-scan", action="store_true", help="skip the line-by-line scanning measurement")
+    ap.add_argument("--bin-split", type=int, default=0,
+                    help="bin split of the headline arm's plan (0: the plan's default tiling)")
     ap.add_argument("--ensemble-frames", type=int, default=400)
     ap.add_argument("--queue-jobs", type=int, default=64)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -204,13 +206,15 @@ def run_saft(steps, warmup, dev, precs):
             "flops_per_frame": info["flops_per_frame"], "arms": res}, w, cols, rf_host
 
 
-def run_arm(w, plan_kw, steps, warmup, dev, keep_image=False):
+def run_arm(w, plan_kw, steps, warmup, dev, keep_image=False, bin_split=0):
     import torch
 
     import f_number_amd as fa
 
     plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
                      w.c_0, w.f_bounds, 0.0, index_gpu=dev.index or 0, **plan_kw)
+    if bin_split > 0:
+        plan.set_tiling(plan.info()["m_tiles"], bin_split)
     info = plan.info()
     rf = w.rf_device(dev)
     img = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device=dev)
@@ -482,7 +486,8 @@ def main():
     hann_kw = dict(window=fa.windows.hann(), F_number=fa.f_numbers.grating.angle_lb(60, 1.5))
     precs = [p for p in args.precisions.split(",") if p]
     if args.arm in ("both", "hann"):
-        arms["hann_angle_lb"] = run_arm(w, hann_kw, args.steps, args.warmup, dev, keep_image=True)
+        arms["hann_angle_lb"] = run_arm(w, hann_kw, args.steps, args.warmup, dev, keep_image=True,
+                                        bin_split=args.bin_split)
     if args.arm in ("both", "boxcar"):
         arms["boxcar_F1.5"] = run_arm(w, dict(window=fa.windows.boxcar(), F_number=fa.f_numbers.constant(1.5)),
                                       args.steps, args.warmup, dev)

@@ -874,6 +874,13 @@ int execute_tangent(fn_plan *P, const void *d_rf, int dtype, float *d_image, flo
 }
 
 // voxel contraction Z_n(k,a) = s sum_v y conj(T B) / w into P->d_Z
+#ifndef FN_ADJ_BLOCKS
+// target workgroups of the adjoint voxel contraction: about sixteen rounds of two blocks
+// per CU (11 angles, 384 x 384, 542 bins: 10.2 ms at 2048, 9.8 at 4096, 9.6 at 8192,
+// 10.8 at 1024; more ranges enlarge the partials the reduction reads, 0.5 GB there)
+#define FN_ADJ_BLOCKS 8192L
+#endif
+
 int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
 {
     const int n_f = P->g.n_f;
@@ -881,9 +888,9 @@ int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
     if (!P->d_Z) {
         P->adj_at = (P->n_tx + 15) / 16;
         const long groups64 = (P->n_vox + 63) / 64;  // ranges are whole 64-voxel aperture blocks
-        // about four rounds of two blocks per CU on 256 CUs (rounded down so the
-        // last round is not a short tail), bounded by the voxel blocks
-        long vr = 2048 / ((long)n_chunks * P->adj_at);
+        // FN_ADJ_BLOCKS workgroups (rounded down so the last round is not a short
+        // tail), bounded by the voxel blocks
+        long vr = FN_ADJ_BLOCKS / ((long)n_chunks * P->adj_at);
         vr = std::max(1L, std::min(vr, groups64));
         const long blocks_per_range = (groups64 + vr - 1) / vr;
         P->adj_vpr = blocks_per_range * 64;

@@ -398,7 +398,7 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
         // a group's band spectra beyond 32 MiB (many transmit tiles, a long band): the bins
         // are split so that the workgroups running together walk about 4 MiB of them (an
         // XCD's L2); at 75 angles x 1455 bins this cut the fetches beyond L2 from 152-179 GB
-        // to 74 GB per launch at the same duration
+        // to 74-93 GB per launch at the same duration
         const double set = (double)g.n_f * (double)P->lds_beam;
         if (set > 32.0 * 1048576.0)
             split = (int)std::min<double>({16.0, std::ceil(set / (4.0 * 1048576.0)), (double)std::max(1, g.n_f / 8)});

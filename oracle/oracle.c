@@ -20,7 +20,16 @@
  *   or_das_pw/or_das_sa  pinned: numpy transliteration of the MATLAB
  *                        statements (oracle/das_numpy.py), single-element /
  *                        single-bin closed forms, linearity, time-shift
- *                        phase, point-target focus, SA/PW reciprocity
+ *                        phase, point-target focus, SA transmit/receive
+ *                        reciprocity
+ *   aperture/apod        pinned: per-element weights extracted from images of
+ *                        unit impulses, against an element-scan construction
+ *                        of the aperture (membership |x_k - x| <= h, half-widths
+ *                        to the outer faces of the outermost members, side from
+ *                        the sign of x_k - x), CPU and MEX semantics, all four
+ *                        windows, normalization on and off, mirror symmetry;
+ *                        the OR_PLANT_* variants below are built by the tests
+ *                        to show that these pins fail on such mistakes
  */
 #include "oracle.h"
 
@@ -405,8 +414,15 @@ static aperture_t aperture(double x, double h, double M, double pitch, int n_el,
 {
     aperture_t ap;
     ap.h = h;
+#ifdef OR_PLANT_CEIL_FLOOR
+    /* planted mistake (tests/test_oracle_aperture.py builds a variant with it and
+       checks that the aperture pins catch it): rounding directions exchanged */
+    const double a = floor(M + (x - h) / pitch);
+    const double b = ceil(M + (x + h) / pitch);
+#else
     const double a = ceil(M + (x - h) / pitch);
     const double b = floor(M + (x + h) / pitch);
+#endif
     const double lbd = fmax(a, 0.0);          /* max( ceil(...) + 1, 1 ) - 1 */
     const double ubd = fmin(b, n_el - 1.0);   /* min( floor(...) + 1, N ) - 1 */
     ap.valid = lbd <= ubd;
@@ -417,8 +433,14 @@ static aperture_t aperture(double x, double h, double M, double pitch, int n_el,
         const double w2 = width / 2;
         const double xlb = ((ap.lb - M) * pitch) - w2; /* positions_lbs_x */
         const double xub = ((ap.ub - M) * pitch) + w2; /* positions_ubs_x */
+#ifdef OR_PLANT_SWAP_W
+        /* planted mistake (see OR_PLANT_CEIL_FLOOR): half-widths exchanged */
+        ap.WL = xub - x;
+        ap.WR = x - xlb;
+#else
         ap.WL = x - xlb;
         ap.WR = xub - x;
+#endif
     }
     return ap;
 }

@@ -11,6 +11,7 @@ das_sa.m:289-509).  Images are returned in the CPU path's scale
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -73,27 +74,51 @@ def build() -> str:
     return _LIB_PATH
 
 
+def build_variant(build_dir: str, plant: str) -> str:
+    """Compile oracle.c with a planted mistake (``-DOR_PLANT_...``) into build_dir.
+
+    Only the aperture tests use this, to show that their pins fail on such a
+    mistake; the default library is never affected."""
+    subprocess.run(["make", "-s", "-C", _HERE, f"BUILD={build_dir}", f"PLANT={plant}"], check=True)
+    return os.path.join(build_dir, "liboracle.so")
+
+
+@contextlib.contextmanager
+def using(path: str):
+    """Route every oracle call inside the block through the library at ``path``."""
+    global _lib
+    saved = _lib
+    _lib = _configure(C.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
-        d = C.POINTER(C.c_double)
-        i = C.POINTER(C.c_int)
-        L.or_grid.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
-                              C.c_double, C.c_double, i, i, i]
-        L.or_fnumber_values.argtypes = [C.POINTER(_FNumber), d, C.c_int, d]
-        L.or_window.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int]
-        L.or_window.restype = C.c_double
-        L.or_spectra.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                 C.c_int, C.c_int, d, d]
-        L.or_das_pw.argtypes = [C.POINTER(_Args)]
-        L.or_das_sa.argtypes = [C.POINTER(_Args)]
-        L.or_das_pw_c.argtypes = [C.POINTER(_Args), C.c_double, d]
-        L.or_das_pw_diff.argtypes = [C.POINTER(_Args), C.c_double, d, C.POINTER(_Diff)]
-        _lib = L
+        _lib = _configure(C.CDLL(_LIB_PATH))
     return _lib
+
+
+def _configure(L):
+    d = C.POINTER(C.c_double)
+    i = C.POINTER(C.c_int)
+    L.or_grid.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                          C.c_double, C.c_double, i, i, i]
+    L.or_fnumber_values.argtypes = [C.POINTER(_FNumber), d, C.c_int, d]
+    L.or_window.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int]
+    L.or_window.restype = C.c_double
+    L.or_spectra.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                             C.c_int, C.c_int, d, d]
+    L.or_das_pw.argtypes = [C.POINTER(_Args)]
+    L.or_das_sa.argtypes = [C.POINTER(_Args)]
+    L.or_das_pw_c.argtypes = [C.POINTER(_Args), C.c_double, d]
+    L.or_das_pw_diff.argtypes = [C.POINTER(_Args), C.c_double, d, C.POINTER(_Diff)]
+    return L
 
 
 def _dp(a):

@@ -27,7 +27,45 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
+PEAK_16BIT_TFLOPS = 2500.0  # MI355X dense BF16 / FP16 matrix
 PEAK_HBM_GBS = 8000.0
+CPU_LABEL = "reference CPU path (FP64 C restatement of das_pw.m / das_sa.m; MATLAB unavailable)"
+
+
+def host_cpus():
+    """(threads for the CPU baseline, CPUs this process may run on, CPU model): every CPU
+    of the process's affinity set, or OMP_NUM_THREADS when that is smaller (the share
+    of a shared host that the process is given)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    n = avail
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, avail, model
+
+
+def precision_roofline(prec, flops, beamform_ms):
+    """Algorithmic and executed MFMA fractions of a precision arm's beamforming kernel:
+    algorithmic = 8 N_f N_vox N_el N_tx / time / peak of the contraction precision;
+    executed counts the MFMA work the form performs (FP32: Gauss's 3 real products per
+    complex MAC, 3/4 of the dense 4; split 16-bit forms: 4 real products x 3 passes; fp16: 4)."""
+    peak = PEAK_FP32_TFLOPS if prec == "fp32" else PEAK_16BIT_TFLOPS
+    executed_per_alg = {"fp32": 0.75, "bf16x3": 3.0, "fp16x3": 3.0, "fp16": 1.0}[prec]
+    achieved = flops / (beamform_ms / 1e3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "executed_frac": achieved * executed_per_alg / peak}
 
 
 def parse():
@@ -63,6 +101,23 @@ def measured_traffic(workload):
             return json.load(f)[workload]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def spectra_roofline(w, n_f, spectra_ms):
+    """The band-limited DFT stage (spectra_kernel): its algorithmic HBM bytes -- the RF
+    read once (4 N_t bytes per channel) and the band spectra written once (8 N_f bytes per
+    channel) -- over its event-timed duration against 8 TB/s, and its DFT arithmetic
+    (4 flop per sample and bin of a real channel) against the FP32 matrix peak, on which
+    it runs."""
+    n_ch = w.n_tx * w.n_elements
+    bytes_alg = n_ch * (4.0 * w.n_samples + 8.0 * n_f)
+    flops = 4.0 * n_ch * w.n_samples * n_f
+    gbs = bytes_alg / (spectra_ms / 1e3) / 1e9
+    tfs = flops / (spectra_ms / 1e3) / 1e12
+    return {"kernel": "spectra_kernel", "bytes_per_launch": bytes_alg, "achieved_gbs": gbs,
+            "hbm_frac": gbs / PEAK_HBM_GBS, "flops_per_launch": flops, "achieved_tflops": tfs,
+            "mfma_frac": tfs / PEAK_FP32_TFLOPS, "avg_launch_ms": spectra_ms,
+            "bound": "mfma" if tfs / PEAK_FP32_TFLOPS > gbs / PEAK_HBM_GBS else "hbm"}
 
 
 def sample_column(w):
@@ -111,10 +166,14 @@ def cpu_baseline(w, window_name, F_kind, F_params, n_threads):
     O.spectra(sub, N, lb, ub, mode=1, n_threads=1)
     t_spec1 = (time.perf_counter() - t0) * ch.shape[0] / sub.shape[0]
     t_frame1 = t_spec1 + t_vox1 * w.n_voxels
+    _, avail, model = host_cpus()
     return {
         "value": w.n_voxels / t_frame / 1e6,
         "unit": "Mvoxel*frames/s",
         "cores": n_threads,
+        "host_cpus": avail,
+        "cpu_model": model,
+        "label": CPU_LABEL,
         "kind": "port",
         "sample": f"spectra of all {w.n_tx * w.n_elements} channels + 1 of {w.positions_x.size} lateral "
                   f"columns ({w.positions_z.size} voxels, all {w.n_tx} angles), extrapolated x{w.positions_x.size}; "
@@ -145,7 +204,9 @@ def saft_fp64_check(w, rf, kw_oracle, n_threads, gpu_cols, n_dft):
     col = np.asarray(col).reshape(-1)
     errs = {k: rel_l2(v * n_dft, col) for k, v in gpu_cols.items()}  # GPU arms: MEX scale 2/N, oracle: 2
     t_frame = t_spec + t_col * w.positions_x.size
-    return errs, {"value": w.n_voxels / t_frame / 1e6, "unit": "Mvoxel*frames/s", "cores": n_threads, "kind": "port",
+    _, avail, model = host_cpus()
+    return errs, {"value": w.n_voxels / t_frame / 1e6, "unit": "Mvoxel*frames/s", "cores": n_threads,
+                  "host_cpus": avail, "cpu_model": model, "label": CPU_LABEL, "kind": "port",
                   "sample": f"spectra of all {w.n_elements ** 2} channels + 1 of {w.positions_x.size} lateral columns "
                             f"({w.positions_z.size} voxels, 128 transmits), extrapolated x{w.positions_x.size}; "
                             f"sample wall {t_spec + t_col:.1f} s"}
@@ -192,6 +253,7 @@ def run_saft(steps, warmup, dev, precs):
         r = {"ms_per_step": wall * 1e3, "beamform_ms": beam_ms, "prep_ms": spec_ms / max(n, 1),
              "value": w.n_voxels / wall / 1e6,
              "achieved_tflops": info["flops_per_frame"] / (beam_ms / 1e3) / 1e12,
+             "roofline": precision_roofline(p, info["flops_per_frame"], beam_ms),
              "finite": bool(torch.isfinite(torch.view_as_real(img)).all().item())}
         if p == "fp32":
             ref = img.clone()
@@ -507,7 +569,8 @@ def main():
             prec_arms[p] = {"ms_per_step": r["ms_per_step"], "beamform_ms": r["beamform_ms"],
                             "prep_ms": r["spectra_ms"], "value": w.n_voxels / (r["ms_per_step"] / 1e3) / 1e6,
                             "beamform_speedup_vs_fp32": head["beamform_ms"] / r["beamform_ms"],
-                            "rel_l2_vs_fp32": err, "finite": r["finite"]}
+                            "rel_l2_vs_fp32": err, "finite": r["finite"],
+                            "roofline": precision_roofline(p, head["info"]["flops_per_frame"], r["beamform_ms"])}
             cols[p] = r["image"][ix_col].cpu().numpy().astype(np.complex128)
             del r
     value = w.n_voxels / (head["ms_per_step"] / 1e3) / 1e6
@@ -548,6 +611,7 @@ def main():
             "avg_launch_ms": head["beamform_ms"],
         },
         "stages_ms": {"spectra": head["spectra_ms"], "beamform": head["beamform_ms"]},
+        "spectra_roofline": spectra_roofline(w, head["info"]["n_f"], head["spectra_ms"]),
         "arms": {k: {"value": w.n_voxels / (v["ms_per_step"] / 1e3) / 1e6, "ms_per_step": v["ms_per_step"],
                      "beamform_ms": v["beamform_ms"], "spectra_ms": v["spectra_ms"], "finite": v["finite"]}
                  for k, v in arms.items()},
@@ -564,7 +628,7 @@ def main():
         saft, w_sa, cols_sa, rf_sa = run_saft(max(2, args.steps // 2), 1, dev, precs)
         out["saft"] = saft
     if not args.no_cpu_baseline:
-        nthreads = min(os.cpu_count() or 1, 16)
+        nthreads = host_cpus()[0]
         if head_key == "hann_angle_lb":
             cb, ref_col = cpu_baseline(w, "hann", "angle_lb", (60, 1.5, 10), nthreads)
             out["cpu_baseline"] = cb

@@ -38,7 +38,7 @@ def _units(extra):
         ("fn_signal.o", [HIPCC, "-x", "hip"] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "fn_signal.hip")]),
     ]
     for mode in (0, 1):
-        for ki in (8, 16, 32):
+        for ki in (8, 16, 32, 48, 64):
             for prec in (0, 1, 2, 3):
                 units.append((f"fn_beamform_m{mode}_k{ki}_p{prec}.o",
                               [HIPCC, "-x", "hip"] + HIP_FLAGS + extra +

@@ -1,6 +1,9 @@
 // fn_beamform_inst.hip -- explicit instantiations of beamform_kernel for one
 // (mode, KI, precision) triple, compiled once per triple (-DINST_MODE=0|1
-// -DINST_KI=8|16|32 -DINST_PREC=0..3) so the instantiation set builds in parallel.
+// -DINST_KI=8|16|32|48|64 -DINST_PREC=0..3) so the instantiation set builds in parallel.
+// KI = 48 and 64 (arrays of up to 192 and 256 elements) provide the one-launch
+// forward kernel only: plane waves in groups of at most 3 / 2 transmit tiles (the
+// staged tile within 80 KiB), synthetic aperture in groups of 4 transmit tiles.
 #include "fn_kernels.hip.h"
 
 #ifndef INST_MODE
@@ -12,6 +15,11 @@
 #ifndef INST_PREC
 #define INST_PREC 0
 #endif
+
+#define FN_WIDE (INST_KI > 32)
+// transmit tiles per plane-wave group: up to 5 (tile_cap); wide arrays while the
+// staged FP32 tile (MT x KI x 512 bytes) stays within 80 KiB
+#define FN_PW_MT_MAX (INST_KI <= 32 ? 5 : 81920 / (INST_KI * 512))
 
 #define FN_CAT6(a, b, c, d, e, f) a##b##c##d##e##f
 #define FN_LAUNCHER(m, k, p) FN_CAT6(launch_beamform_m, m, _k, k, _p, p)
@@ -61,7 +69,7 @@ static hipError_t launch_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, const
     }
 }
 
-#if INST_PREC == 0
+#if INST_PREC == 0 && !FN_WIDE
 // tangent (dI / dc_tof) variant: groups of at most two transmit tiles
 #define FN_TAN_LAUNCHER(k) FN_CAT4(launch_beamform_tan, _k, k, )
 #define FN_CAT4(a, b, c, d) a##b##c##d
@@ -115,7 +123,7 @@ hipError_t FN_LBL_LAUNCHER(INST_KI)(int wcr, dim3 grid, hipStream_t s, const Bea
 }
 #endif
 
-#if INST_MODE == 0
+#if INST_MODE == 0 && !FN_WIDE
 // ensemble variant: frames x angles as transmit rows, up to four tiles per group
 #define FN_ENS_LAUNCHER(k, p) FN_CAT6(launch_beamform_ens, _k, k, _p, p, )
 template <int MT, int WCR>
@@ -160,18 +168,24 @@ hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, 
                                             const BeamArgs &A)
 {
     (void)wct;
+    if (mt < 1 || mt > FN_PW_MT_MAX) return hipErrorInvalidValue;
     switch (mt) {
     case 1: return launch_mt<1>(wcr, grid, lds, s, A);
     case 2: return launch_mt<2>(wcr, grid, lds, s, A);
+#if FN_PW_MT_MAX >= 3
     case 3: return launch_mt<3>(wcr, grid, lds, s, A);
+#endif
+#if FN_PW_MT_MAX >= 5
     case 4: return launch_mt<4>(wcr, grid, lds, s, A);
     case 5: return launch_mt<5>(wcr, grid, lds, s, A);  // at most 5 tiles per group (tile_cap)
+#endif
     default: return hipErrorInvalidValue;
     }
 }
 #else
-// synthetic aperture: transmit rows = elements, MT = 4 KI / 16 = KI / 4 tiles
-constexpr int SA_MT = INST_KI / 4;
+// synthetic aperture: transmit rows = elements, MT = 4 KI / 16 = KI / 4 tiles (all
+// transmitters in one group); wide arrays: groups of 4 tiles
+constexpr int SA_MT = INST_KI <= 32 ? INST_KI / 4 : 4;
 template <int WCR>
 static hipError_t launch_wct(int wct, dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
@@ -196,6 +210,7 @@ hipError_t FN_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, int wct, 
 }
 #endif
 
+#if !FN_WIDE
 // persistent batch launches: one kernel per class over the work items of many
 // plans (grid <= 0: only report the resident workgroups per CU in *occupancy)
 #define FN_PERSIST_LAUNCHER(m, k, p) FN_CAT6(launch_beamform_persistent_m, m, _k, k, _p, p)
@@ -255,7 +270,9 @@ hipError_t FN_PERSIST_LAUNCHER(INST_MODE, INST_KI, INST_PREC)(int mt, int wcr, i
 #endif
 }
 
-#if INST_PREC == 0
+#endif  // !FN_WIDE
+
+#if INST_PREC == 0 && !FN_WIDE
 // windowed-normalization variant (FP32, general window classes): plane-wave
 // groups of at most two transmit tiles, synthetic aperture at its fixed tiling
 #define FN_NW_LAUNCHER(m, k) FN_CAT6(launch_beamform_nw_m, m, _k, k, , )

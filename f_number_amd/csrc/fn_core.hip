@@ -27,6 +27,10 @@ FN_DECL_LAUNCHERS(0, 32)
 FN_DECL_LAUNCHERS(1, 8)
 FN_DECL_LAUNCHERS(1, 16)
 FN_DECL_LAUNCHERS(1, 32)
+FN_DECL_LAUNCHERS(0, 48)
+FN_DECL_LAUNCHERS(0, 64)
+FN_DECL_LAUNCHERS(1, 48)
+FN_DECL_LAUNCHERS(1, 64)
 #undef FN_DECL_LAUNCHERS
 #define FN_DECL_ENS(k)                                                                              \
     hipError_t launch_beamform_ens_k##k##_p0(int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
@@ -89,17 +93,17 @@ hipError_t launch_vjp_delay(const float2 *X, const float2 *Z, double *out, int n
                             int n_lb, double w_unit, hipStream_t s);
 hipError_t launch_dot_re(const float2 *g, const float2 *d, long n, double *part, int n_part, double *out, hipStream_t s);
 typedef hipError_t (*BeamLauncher)(int, int, int, dim3, size_t, hipStream_t, const BeamArgs &);
-// [mode][KI = 8, 16, 32][precision]
+// [mode][KI = 8, 16, 32] (windowed normalization; [mode][KI = 8 .. 64][precision] below)
 static const BeamLauncher kNwLaunchers[2][3] = {
     {launch_beamform_nw_m0_k8, launch_beamform_nw_m0_k16, launch_beamform_nw_m0_k32},
     {launch_beamform_nw_m1_k8, launch_beamform_nw_m1_k16, launch_beamform_nw_m1_k32}};
-static const BeamLauncher kBeamLaunchers[2][3][4] = {
-    {{launch_beamform_m0_k8_p0, launch_beamform_m0_k8_p1, launch_beamform_m0_k8_p2, launch_beamform_m0_k8_p3},
-     {launch_beamform_m0_k16_p0, launch_beamform_m0_k16_p1, launch_beamform_m0_k16_p2, launch_beamform_m0_k16_p3},
-     {launch_beamform_m0_k32_p0, launch_beamform_m0_k32_p1, launch_beamform_m0_k32_p2, launch_beamform_m0_k32_p3}},
-    {{launch_beamform_m1_k8_p0, launch_beamform_m1_k8_p1, launch_beamform_m1_k8_p2, launch_beamform_m1_k8_p3},
-     {launch_beamform_m1_k16_p0, launch_beamform_m1_k16_p1, launch_beamform_m1_k16_p2, launch_beamform_m1_k16_p3},
-     {launch_beamform_m1_k32_p0, launch_beamform_m1_k32_p1, launch_beamform_m1_k32_p2, launch_beamform_m1_k32_p3}}};
+#define FN_BEAM_ROW(m, k) \
+    {launch_beamform_m##m##_k##k##_p0, launch_beamform_m##m##_k##k##_p1, launch_beamform_m##m##_k##k##_p2, \
+     launch_beamform_m##m##_k##k##_p3}
+static const BeamLauncher kBeamLaunchers[2][5][4] = {
+    {FN_BEAM_ROW(0, 8), FN_BEAM_ROW(0, 16), FN_BEAM_ROW(0, 32), FN_BEAM_ROW(0, 48), FN_BEAM_ROW(0, 64)},
+    {FN_BEAM_ROW(1, 8), FN_BEAM_ROW(1, 16), FN_BEAM_ROW(1, 32), FN_BEAM_ROW(1, 48), FN_BEAM_ROW(1, 64)}};
+#undef FN_BEAM_ROW
 // zero fill used instead of hipMemsetAsync so the launch sequence stays
 // capturable into a graph
 __global__ void fill_zero_kernel(float *p, long n)
@@ -160,6 +164,10 @@ struct fn_plan {
     hipStream_t adj_sc_stream = nullptr;
     long adj_vpr = 0;
     size_t x_bytes = 0, xlp_bytes = 0;
+    float2 *d_pimg = nullptr;   // partial images [slot][n_vox] (several workgroups per voxel)
+    size_t pimg_cap = 0;        // float2 capacity of d_pimg
+    float2 *d_ptan = nullptr;   // tangent partials: image slots, then dI/dc slots
+    size_t ptan_cap = 0;
     hipStream_t stream = nullptr;
     int profiling = 0;
     std::vector<Run> pending, spare;
@@ -173,6 +181,10 @@ namespace {
 // voxel tiles of 64 per beamforming group: the whole grid, or (line scan) one column
 long beam_tiles(const fn_plan *P) { return P->lbl ? (P->n_z + 63) / 64 : (P->n_vox + 63) / 64; }
 
+// workgroups that share a voxel: bin chunks times transmit groups (a line scan's
+// groups are its columns, one per voxel); ensembles add their frames' rows atomically
+int beam_slots(const fn_plan *P) { return P->n_frames > 1 ? 1 : P->bin_split * (P->lbl ? 1 : P->n_groups); }
+
 int hip_fail(hipError_t e, const char *what)
 {
     if (e == hipErrorOutOfMemory) return fail(FN_ERR_OUT_OF_MEMORY, "%s: out of device memory", what);
@@ -184,6 +196,21 @@ int hip_fail(hipError_t e, const char *what)
         hipError_t e_ = (call);                        \
         if (e_ != hipSuccess) return hip_fail(e_, what); \
     } while (0)
+
+// partial-image buffer of the plan's tiling (grown, never shrunk; freed with the plan)
+int ensure_partials(fn_plan *P)
+{
+    const int slots = beam_slots(P);
+    const size_t need = slots > 1 ? (size_t)slots * P->n_vox : 0;
+    if (need <= P->pimg_cap) return FN_OK;
+    HIP_CHECK(hipSetDevice(P->device), "hipSetDevice");
+    if (P->d_pimg) HIP_CHECK(hipFree(P->d_pimg), "hipFree");  // synchronizes: no launch still reads it
+    P->d_pimg = nullptr;
+    P->pimg_cap = 0;
+    HIP_CHECK(hipMalloc(&P->d_pimg, need * sizeof(float2)), "partial images");
+    P->pimg_cap = need;
+    return FN_OK;
+}
 
 int select_device(int index)
 {
@@ -308,13 +335,27 @@ int tile_cap(int n_frames, int precision)
     return (precision == FN_PRECISION_BF16X3 || precision == FN_PRECISION_FP16X3) ? 3 : 4;
 }
 
+// element steps per lane: 4 KI >= n_el elements in four lane groups
 int ki_of(int n_el)
 {
     if (n_el <= 32) return 8;
     if (n_el <= 64) return 16;
     if (n_el <= 128) return 32;
+    if (n_el <= 192) return 48;
+    if (n_el <= 256) return 64;
     return 0;
 }
+
+// index of a KI in the launcher tables
+int ki_index(int KI) { return KI == 8 ? 0 : KI == 16 ? 1 : KI == 32 ? 2 : KI == 48 ? 3 : 4; }
+
+// transmit tiles per synthetic-aperture group: every transmitter up to 128
+// elements, groups of 4 tiles beyond (the staged tile within 128 KiB)
+int sa_mt(int KI) { return KI <= 32 ? KI / 4 : 4; }
+
+// transmit tiles per plane-wave group beyond 128 elements: the staged FP32 tile
+// (MT x KI x 512 bytes) within 80 KiB
+int pw_mt_wide(int KI) { return KI <= 32 ? 5 : 81920 / (KI * 512); }
 
 struct Common {
     const double *px, *pz;
@@ -329,7 +370,7 @@ void destroy(fn_plan *p)
     hipFree(p->d_px); hipFree(p->d_pz); hipFree(p->d_Frx); hipFree(p->d_Ftx); hipFree(p->d_txp); hipFree(p->d_sqf);
     hipFree(p->d_tw); hipFree(p->d_X); hipFree(p->d_Xlp); hipFree(p->d_xscale);
     hipFree(p->d_delay); hipFree(p->d_P); hipFree(p->d_Z); hipFree(p->d_part); hipFree(p->d_qtab); hipFree(p->d_vsc); hipFree(p->d_nw);
-    hipFree(p->d_sigF);
+    hipFree(p->d_sigF); hipFree(p->d_pimg); hipFree(p->d_ptan);
     for (auto &r : p->pending) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     for (auto &r : p->spare) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); hipEventDestroy(r.e2); }
     if (p->stream) hipStreamDestroy(p->stream);
@@ -344,13 +385,13 @@ int upload(T **dst, const T *src, size_t n, const char *what)
     return FN_OK;
 }
 
-// shared part of plan creation
-int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double *Frx, const double *Ftx)
+// shared configuration of plan creation: tiling, launch shape and the scalar
+// arguments of the kernels; no device work and no allocation (the
+// no-allocation entry point runs it on a plan held on the stack)
+int configure_common(fn_plan *P, const Common &c, const fnh::Grid &g)
 {
     if (c.precision < FN_PRECISION_FP32 || c.precision > FN_PRECISION_FP16)
         return fail(FN_ERR_ARGUMENT, "unknown precision %d", c.precision);
-    int s = select_device(c.index_gpu);
-    if (s) return s;
     P->device = c.index_gpu;
     P->g = g;
     P->n_x = c.n_x; P->n_z = c.n_z; P->n_t = c.n_t; P->n_el = c.n_el; P->n_tx = c.n_tx;
@@ -358,7 +399,15 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     P->verbosity = c.verbosity;
     P->precision = c.precision;
     P->KI = ki_of(c.n_el);
-    if (!P->KI) return fail(FN_ERR_UNSUPPORTED, "at most 128 array elements are supported (got %d)", c.n_el);
+    if (!P->KI) return fail(FN_ERR_UNSUPPORTED, "at most 256 array elements are supported (got %d)", c.n_el);
+    if (P->KI > 32) {
+        // beyond 128 elements: the one-launch forward kernel of single-frame plans
+        if (P->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "ensembles: at most 128 elements (got %d)", c.n_el);
+        if (P->lbl) return fail(FN_ERR_UNSUPPORTED, "line scans: at most 128 elements (got %d)", c.n_el);
+        if (P->nw)
+            return fail(FN_ERR_UNSUPPORTED, "normalizations.windowed (CPU semantics): at most 128 elements (got %d)",
+                        c.n_el);
+    }
     P->MT_all = (c.n_tx + 15) / 16;
     if (P->lbl) {
         // line scan: one group per grid column, staging the M-tile of its transmit row
@@ -367,13 +416,13 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     } else if (P->mode == 0) {
         // at most 5 transmit tiles per group keep the LDS tile within 80 KiB
         // (ensembles: 4, for the per-row image accumulators)
-        const int max_tiles = P->nw ? 2 : tile_cap(P->n_frames, c.precision);
+        const int max_tiles = P->nw ? 2 : std::min(tile_cap(P->n_frames, c.precision), pw_mt_wide(P->KI));
         const int groups = (P->MT_all + max_tiles - 1) / max_tiles;
         P->MT_tmpl = (P->MT_all + groups - 1) / groups;
         P->n_groups = (P->MT_all + P->MT_tmpl - 1) / P->MT_tmpl;
     } else {
-        P->MT_tmpl = P->KI / 4;
-        P->n_groups = 1;
+        P->MT_tmpl = sa_mt(P->KI);
+        P->n_groups = (P->MT_all + P->MT_tmpl - 1) / P->MT_tmpl;
     }
     // staged spectra per bin: FP32 interleaved complex (8 bytes per value) or
     // 16-bit planes (Xr, Xi and, for the split forms, their residuals)
@@ -405,6 +454,35 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     }
     const int chunk = (g.n_f + split - 1) / split;
     P->bin_split = (g.n_f + chunk - 1) / chunk;
+    P->x_bytes = (size_t)g.n_f * P->MT_all * P->KI * 64 * sizeof(float2);
+    if (c.precision != FN_PRECISION_FP32) P->xlp_bytes = (size_t)g.n_f * planes * P->MT_all * P->KI * 64 * 2;
+
+    fnk::BeamArgs &A = P->args;
+    A.ctof = c.c0;
+    A.dimage = nullptr;
+    A.n_x = c.n_x; A.n_z = c.n_z; A.n_el = c.n_el; A.n_tx = c.n_tx;
+    A.n_f = g.n_f; A.n_lb = g.n_lb; A.N = g.N;
+    A.bin_chunk = chunk;
+    A.tx_group = P->MT_tmpl;
+    A.M = (c.n_el - 1) / 2.0;
+    A.pitch = c.pitch; A.width = c.width; A.c0 = c.c0; A.f_s = c.f_s; A.t0 = c.t0;
+    // das_pw.m:384,467 (CPU: 2 * sum) vs gpu_bf_das_pw_rf.cu:1118-1119 (MEX: 2 / N * sum)
+    A.out_scale = (c.scale == FN_SCALE_CPU) ? 2.0f : (float)(2.0 / g.N);
+
+    fnk::SpecArgs &S = P->sargs;
+    S.n_t = c.n_t; S.n_ch = c.n_el * c.n_tx; S.N = g.N; S.n_lb = g.n_lb; S.n_f = g.n_f;
+    S.n_el = c.n_el; S.n_tx = c.n_tx; S.MT = P->MT_all; S.KI = P->KI;
+
+    P->flops = 8.0 * g.n_f * (double)P->n_vox * c.n_el * (P->lbl ? 1 : c.n_tx);
+    return FN_OK;
+}
+
+// shared part of plan creation: configuration, then the device tables and workspace
+int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double *Frx, const double *Ftx)
+{
+    int s = configure_common(P, c, g);
+    if (s) return s;
+    if ((s = select_device(c.index_gpu))) return s;
 
     HIP_CHECK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate");
     if ((s = upload(&P->d_px, c.px, c.n_x, "positions_x"))) return s;
@@ -427,11 +505,9 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
         tw[m] = make_float2((float)cosl(ang), (float)sinl(ang));
     }
     if ((s = upload(&P->d_tw, tw.data(), tw.size(), "twiddles"))) return s;
-    P->x_bytes = (size_t)g.n_f * P->MT_all * P->KI * 64 * sizeof(float2);
     HIP_CHECK(hipMalloc(&P->d_X, P->x_bytes), "spectra workspace");
     HIP_CHECK(hipMemset(P->d_X, 0, P->x_bytes), "spectra workspace");  // padding rows stay zero
     if (c.precision != FN_PRECISION_FP32) {
-        P->xlp_bytes = (size_t)g.n_f * planes * P->MT_all * P->KI * 64 * 2;
         HIP_CHECK(hipMalloc(&P->d_Xlp, P->xlp_bytes), "16-bit spectra planes");
         HIP_CHECK(hipMalloc(&P->d_xscale, (size_t)g.n_f * sizeof(float)), "spectra scales");
     }
@@ -442,27 +518,10 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     fnk::BeamArgs &A = P->args;
     A.X = P->d_X;
     A.delay = P->d_delay;
-    A.ctof = c.c0;
-    A.dimage = nullptr;
     A.Xlp = P->d_Xlp;
     A.xscale = P->d_xscale;
     A.px = P->d_px; A.pz = P->d_pz; A.Frx = P->d_Frx; A.Ftx = P->d_Ftx; A.txp = P->d_txp; A.sqrt_f = P->d_sqf;
-    A.n_x = c.n_x; A.n_z = c.n_z; A.n_el = c.n_el; A.n_tx = c.n_tx;
-    A.n_f = g.n_f; A.n_lb = g.n_lb; A.N = g.N;
-    A.bin_chunk = chunk;
-    A.tx_group = P->MT_tmpl;
-    A.atomic_out = (P->bin_split > 1 || (P->n_groups > 1 && !P->lbl)) ? 1 : 0;  // line scan: a column per group
-    A.M = (c.n_el - 1) / 2.0;
-    A.pitch = c.pitch; A.width = c.width; A.c0 = c.c0; A.f_s = c.f_s; A.t0 = c.t0;
-    // das_pw.m:384,467 (CPU: 2 * sum) vs gpu_bf_das_pw_rf.cu:1118-1119 (MEX: 2 / N * sum)
-    A.out_scale = (c.scale == FN_SCALE_CPU) ? 2.0f : (float)(2.0 / g.N);
-
-    fnk::SpecArgs &S = P->sargs;
-    S.n_t = c.n_t; S.n_ch = c.n_el * c.n_tx; S.N = g.N; S.n_lb = g.n_lb; S.n_f = g.n_f;
-    S.n_el = c.n_el; S.n_tx = c.n_tx; S.MT = P->MT_all; S.KI = P->KI;
-
-    P->flops = 8.0 * g.n_f * (double)P->n_vox * c.n_el * (P->lbl ? 1 : c.n_tx);
-    return FN_OK;
+    return ensure_partials(P);
 }
 
 // useful (valid voxel, bin) count of the receive aperture, for the roofline
@@ -489,7 +548,7 @@ double useful_fraction(const fnh::Grid &g, const double *px, int nx, const doubl
 
 hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipStream_t s)
 {
-    const int kidx = P->KI == 8 ? 0 : (P->KI == 16 ? 1 : 2);
+    const int kidx = ki_index(P->KI);  // ensembles, line scans and windowed plans stop at KI = 32
     if (P->lbl) return fnk::kLblLaunchers[kidx](P->wcr, grid, s, A);
     if (P->nw) return fnk::kNwLaunchers[P->mode][kidx](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
     if (P->n_frames > 1) return fnk::kEnsLaunchers[kidx][P->precision](P->MT_tmpl, P->wcr, grid, P->lds_beam, s, A);
@@ -528,18 +587,14 @@ int launch_spectra(fn_plan *P, const void *d_rf, int dtype, hipStream_t s)
     return FN_OK;
 }
 
-int fill_zero(float *p, long n, hipStream_t s)
-{
-    hipLaunchKernelGGL(fnk::fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
-    HIP_CHECK(hipGetLastError(), "fill_zero_kernel");
-    return FN_OK;
-}
-
 // the beamforming arguments of one execute (image and weight buffers)
 fnk::BeamArgs beam_args(const fn_plan *P, float *d_image, float *d_w, float *d_w_rx)
 {
     fnk::BeamArgs A = P->args;
     A.image = reinterpret_cast<float2 *>(d_image);
+    A.n_slots = beam_slots(P);
+    A.slot_z = P->lbl ? 1 : P->n_groups;
+    A.partial = A.n_slots > 1 ? P->d_pimg : nullptr;
     if (P->mode == 0) {
         A.weights = d_w;
         A.weights_tx = nullptr;
@@ -550,11 +605,23 @@ fnk::BeamArgs beam_args(const fn_plan *P, float *d_image, float *d_w, float *d_w
     return A;
 }
 
-// everything of an execute before the beamforming launch: the zeroed image for
-// atomic accumulation, the band spectra and (16-bit forms) their planes
+// after the beamforming launch: the partial images of a voxel's workgroups summed
+// in slot order into the image (nothing when one workgroup owns each voxel)
+int finish_beam(const fn_plan *P, const fnk::BeamArgs &A, hipStream_t s, float2 *dimage = nullptr)
+{
+    if (!A.partial) return FN_OK;
+    hipLaunchKernelGGL(fnk::reduce_partials_kernel, dim3((unsigned)((P->n_vox + 255) / 256)), dim3(256), 0, s,
+                       A.partial, A.n_slots, P->n_vox, A.image, dimage);
+    HIP_CHECK(hipGetLastError(), "reduce_partials_kernel");
+    return FN_OK;
+}
+
+// everything of an execute before the beamforming launch: the zeroed images of an
+// ensemble (its frames' rows are added atomically), the band spectra and (16-bit
+// forms) their planes
 int prepare(fn_plan *P, const void *d_rf, int dtype, float *d_image, hipStream_t s)
 {
-    if (P->args.atomic_out || P->n_frames > 1) {
+    if (P->n_frames > 1) {
         const long n = 2 * P->n_vox * P->n_frames;
         hipLaunchKernelGGL(fnk::fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_image, n);
         HIP_CHECK(hipGetLastError(), "fill_zero_kernel");
@@ -603,6 +670,7 @@ int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w,
     const long tiles = beam_tiles(P);
     hipError_t e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups), A, s);
     if (e != hipSuccess) return hip_fail(e, "beamform_kernel");
+    if (int st = finish_beam(P, A, s)) return st;
     if (P->profiling) {
         HIP_CHECK(hipEventRecord(run.e2, s), "hipEventRecord");
         P->pending.push_back(run);
@@ -626,13 +694,10 @@ int collect_profile(fn_plan *P)
     return FN_OK;
 }
 
-int pw_plan(const fn_pw_params *p, fn_plan **out)
+// plane-wave configuration shared by plan creation and the no-allocation entry
+// point: receive window, normalization, transmit rows (no device work)
+int configure_pw(fn_plan *P, const fn_pw_params *p, Common *c)
 {
-    fnh::Grid g;
-    int s = fnh::validate_pw(p, &g);
-    if (s) return s;
-    fn_plan *P = new (std::nothrow) fn_plan();
-    if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
     P->mode = 0;
     P->n_frames = std::max(1, p->n_frames);
     P->lbl = p->line_scan != 0;
@@ -640,22 +705,35 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
     // reference MEX reads the window and normalizes as "on" (gpu_bf_das_pw_rf.cu:421-427)
     P->nw = p->normalization == FN_NORMALIZATION_WINDOWED && p->semantics == FN_SEMANTICS_CPU;
     int wsup = 0;
-    if ((s = setup_window(p->window, p->semantics, &P->args.wrx, &P->wcr, &wsup))) { destroy(P); return s; }
+    if (int s = setup_window(p->window, p->semantics, &P->args.wrx, &P->wcr, &wsup)) return s;
     P->args.support_rx = wsup;
     P->args.norm_rx = p->normalization != FN_NORMALIZATION_OFF;
     P->args.clamp = p->semantics == FN_SEMANTICS_CPU;
     // transmit rows: the angles of every frame (row = frame * n_angles + angle)
     const int n_rows = p->n_angles * P->n_frames;
     P->args.rows_per_frame = P->n_frames > 1 ? p->n_angles : 0;
-    Common c{p->positions_x, p->positions_z, p->n_x, p->n_z, p->n_samples, p->n_elements, n_rows,
-             p->f_s, p->element_width, p->element_pitch, p->c_0, p->index_t0,
-             p->semantics, p->precision, p->output_scale, p->index_gpu, p->verbosity};
+    P->args.tx_kind = p->virtual_sources ? fnk::TX_VIRTUAL : fnk::TX_PLANE;
+    *c = Common{p->positions_x, p->positions_z, p->n_x, p->n_z, p->n_samples, p->n_elements, n_rows,
+                p->f_s, p->element_width, p->element_pitch, p->c_0, p->index_t0,
+                p->semantics, p->precision, p->output_scale, p->index_gpu, p->verbosity};
+    return FN_OK;
+}
+
+int pw_plan(const fn_pw_params *p, fn_plan **out)
+{
+    fnh::Grid g;
+    int s = fnh::validate_pw(p, &g);
+    if (s) return s;
+    fn_plan *P = new (std::nothrow) fn_plan();
+    if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
+    Common c{};
+    if ((s = configure_pw(P, p, &c))) { destroy(P); return s; }
+    const int n_rows = c.n_tx;
     if ((s = create_common(P, c, g, p->f_number_values, nullptr))) { destroy(P); return s; }
     // steering geometry in FP64: das_pw.m:323-327 (x_ref = sign(e_x) x_0), or the
     // virtual sources (x_vs, z_vs, |r_vs|)
     std::vector<double> txp(3 * (size_t)n_rows);
     const double M = (p->n_elements - 1) / 2.0;
-    P->args.tx_kind = p->virtual_sources ? fnk::TX_VIRTUAL : fnk::TX_PLANE;
     for (int row = 0; row < n_rows; ++row) {
         const int a = row % p->n_angles;
         if (p->virtual_sources) {
@@ -723,16 +801,16 @@ size_t dtype_size(int dtype) { return dtype == FN_DTYPE_F64 ? 8 : 4; }
 
 int max_tiles(const fn_plan *P)
 {
-    if (P->mode == 1) return P->KI / 4;
+    if (P->mode == 1) return sa_mt(P->KI);
     if (P->lbl) return 1;
-    return std::min(P->MT_all, P->nw ? 2 : tile_cap(P->n_frames, P->precision));
+    return std::min(P->MT_all, P->nw ? 2 : std::min(tile_cap(P->n_frames, P->precision), pw_mt_wide(P->KI)));
 }
 
 int set_tiling(fn_plan *P, int mt, int split)
 {
     if (!P) return fail(FN_ERR_ARGUMENT, "null plan");
     if (P->mode == 1) {
-        if (mt != P->KI / 4) return fail(FN_ERR_ARGUMENT, "synthetic-aperture plans use %d transmit tiles", P->KI / 4);
+        if (mt != sa_mt(P->KI)) return fail(FN_ERR_ARGUMENT, "synthetic-aperture plans use %d transmit tiles", sa_mt(P->KI));
     } else if (mt < 1 || mt > max_tiles(P)) {
         return fail(FN_ERR_ARGUMENT, "m_tiles must be in 1..%d", max_tiles(P));
     }
@@ -745,13 +823,12 @@ int set_tiling(fn_plan *P, int mt, int split)
     if (lds > kMaxLds) return fail(FN_ERR_ARGUMENT, "transmit tile exceeds LDS");
     const int chunk = (n_f + split - 1) / split;
     P->MT_tmpl = mt;
-    P->n_groups = P->mode == 1 ? 1 : (P->lbl ? P->n_x : (P->MT_all + mt - 1) / mt);
+    P->n_groups = P->lbl ? P->n_x : (P->MT_all + mt - 1) / mt;
     P->bin_split = (n_f + chunk - 1) / chunk;
     P->lds_beam = lds;
     P->args.tx_group = mt;
     P->args.bin_chunk = chunk;
-    P->args.atomic_out = (P->bin_split > 1 || (P->n_groups > 1 && !P->lbl)) ? 1 : 0;
-    return FN_OK;
+    return ensure_partials(P);
 }
 
 int autotune(fn_plan *P, const void *d_rf, int dtype, float *d_image, int reps, fn_tune_result *res, hipStream_t s)
@@ -761,7 +838,7 @@ int autotune(fn_plan *P, const void *d_rf, int dtype, float *d_image, int reps, 
     const int mt0 = P->MT_tmpl, split0 = P->bin_split;
     std::vector<std::pair<int, int>> cand;
     const int n_f = P->g.n_f;
-    const int mt_lo = P->mode == 1 ? P->KI / 4 : 1, mt_hi = P->mode == 1 ? P->KI / 4 : max_tiles(P);
+    const int mt_lo = P->mode == 1 ? sa_mt(P->KI) : 1, mt_hi = max_tiles(P);
     for (int mt = mt_lo; mt <= mt_hi; ++mt) {
         // distinct group counts only: the tile count of a balanced split
         if (P->mode == 0) {
@@ -869,10 +946,20 @@ int execute_tangent(fn_plan *P, const void *d_rf, int dtype, float *d_image, flo
     const int chunk = (P->g.n_f + split - 1) / split;
     split = (P->g.n_f + chunk - 1) / chunk;
     A.bin_chunk = chunk;
-    A.atomic_out = (split > 1 || groups > 1) ? 1 : 0;
-    if (A.atomic_out) {
-        if (int st = fill_zero(d_image, 2 * P->n_vox, s)) return st;
-        if (int st = fill_zero(d_dimage, 2 * P->n_vox, s)) return st;
+    // partial images and tangents of the workgroups that share a voxel, summed in slot order
+    A.n_slots = split * groups;
+    A.slot_z = groups;
+    A.partial = nullptr;
+    if (A.n_slots > 1) {
+        const size_t need = 2 * (size_t)A.n_slots * P->n_vox;
+        if (need > P->ptan_cap) {
+            if (P->d_ptan) HIP_CHECK(hipFree(P->d_ptan), "hipFree");
+            P->d_ptan = nullptr;
+            P->ptan_cap = 0;
+            HIP_CHECK(hipMalloc(&P->d_ptan, need * sizeof(float2)), "tangent partial images");
+            P->ptan_cap = need;
+        }
+        A.partial = P->d_ptan;
     }
     const size_t lds = (size_t)mt * P->KI * 64 * sizeof(float2);
     const dim3 grid((unsigned)tiles, (unsigned)split, (unsigned)groups);
@@ -881,7 +968,7 @@ int execute_tangent(fn_plan *P, const void *d_rf, int dtype, float *d_image, flo
                  : P->KI == 16 ? fnk::launch_beamform_tan_k16(mt, wcr, grid, lds, s, A)
                                : fnk::launch_beamform_tan_k32(mt, wcr, grid, lds, s, A);
     if (e != hipSuccess) return hip_fail(e, "beamform tangent kernel");
-    return FN_OK;
+    return finish_beam(P, A, s, reinterpret_cast<float2 *>(d_dimage));
 }
 
 // voxel contraction Z_n(k,a) = s sum_v y conj(T B) / w into P->d_Z
@@ -1008,6 +1095,7 @@ int focused_signal(fn_plan *P, float *d_signal, hipStream_t s)
     if (!P || !d_signal) return fail(FN_ERR_ARGUMENT, "null argument");
     if (P->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "focused signal: single-frame plans only");
     if (P->lbl) return fail(FN_ERR_UNSUPPORTED, "focused signal: not available for line-scan plans");
+    if (P->n_el > 128) return fail(FN_ERR_UNSUPPORTED, "focused signal: at most 128 elements");
     if (!P->executed) return fail(FN_ERR_ARGUMENT, "focused signal: execute the plan first");
     int vx = -1, vz = -1;  // last lateral position with a valid bin, its last such depth
     for (int ix = P->n_x - 1; ix >= 0 && vx < 0; --ix)
@@ -1292,7 +1380,7 @@ static void batch_release_graph(fn_batch *b)
 // (ensembles, windowed normalization)
 static long batch_class_key(const fn_plan *P)
 {
-    if (P->n_frames > 1 || P->nw || P->lbl) return -1;
+    if (P->n_frames > 1 || P->nw || P->lbl || P->KI > 32) return -1;  // persistent kernels up to 128 elements
     return ((((long)P->mode * 64 + P->KI) * 16 + P->MT_tmpl) * 8 + P->wcr) * 8 * 4 + P->wct * 4 + P->precision;
 }
 
@@ -1435,9 +1523,9 @@ int32_t fn_batch_execute(fn_batch *b, const void *const *d_rf, const int32_t *dt
             hipStream_t ws = b->streams[slot++ % W];
             st = prepare(P, d_rf[i], dtypes[i], d_images[i], ws);
             const long tiles = beam_tiles(P);
-            if (!st)
-                e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups),
-                                beam_args(P, d_images[i], nullptr, nullptr), ws);
+            const fnk::BeamArgs A = beam_args(P, d_images[i], nullptr, nullptr);
+            if (!st) e = launch_beam(P, dim3((unsigned)tiles, (unsigned)P->bin_split, (unsigned)P->n_groups), A, ws);
+            if (!st && e == hipSuccess) st = finish_beam(P, A, ws);
         }
         const size_t w0 = n_single < W ? n_single : 0, n_w = n_single < W ? W - n_single : W;
         slot = 0;
@@ -1462,6 +1550,10 @@ int32_t fn_batch_execute(fn_batch *b, const void *const *d_rf, const int32_t *dt
             e = fnk::kPersistLaunchers[P0->mode][kidx][P0->precision](P0->MT_tmpl, P0->wcr, P0->wct, c.grid,
                                                                      P0->lds_beam, b->pstream, c.d_jobs, c.d_items,
                                                                      c.n_items, c.d_counter, nullptr);
+            for (int i : c.plans) {
+                if (e != hipSuccess || st) break;
+                st = finish_beam(b->plans[i], beam_args(b->plans[i], d_images[i], nullptr, nullptr), b->pstream);
+            }
         }
         for (size_t w = 0; w < W && e == hipSuccess && !st; ++w) {
             e = hipEventRecord(b->joins[w], b->streams[w]);

@@ -84,7 +84,12 @@ struct BeamArgs {
     int clamp;              // CPU-semantics depth clamp of the PW F-number (das_pw.m:374-377)
     int support_rx, support_tx;        // SUPPORT_* of the receive / transmit window
     int norm_rx, norm_tx;
-    int atomic_out;
+    // partial images: non-null when several workgroups share a voxel (bin chunks,
+    // transmit groups): workgroup (by, bz) writes slot by * slot_z + bz of
+    // [n_slots][n_vox] (the tangent: dI/dc in slots n_slots .. 2 n_slots - 1), and
+    // reduce_partials_kernel sums the slots in a fixed order (reproducible images)
+    float2 *partial;
+    int slot_z, n_slots;
     double M, pitch, width, c0, f_s, t0;
     int rows_per_frame;     // ensemble: transmit rows per frame (angles); images [frame][n_vox]
     int tx_kind;            // TX_PLANE: txp = (e_x, e_z, x_ref); TX_VIRTUAL: txp = (x_vs, z_vs, |r_vs|)
@@ -237,8 +242,7 @@ __device__ __forceinline__ float window_eval(float t, const WinCoef &w)
     if (WC == WC_LINEAR) {
         v = w.A + w.C * t;
     } else {
-        float u = (t - w.t_off) * w.s_half;   // revolutions
-        u = u - rintf(u);                     // keep v_cos_f32 in its reduced range
+        const float u = __builtin_amdgcn_fractf((t - w.t_off) * w.s_half);  // revolutions, reduced to [0, 1)
         v = w.A + w.B * __builtin_amdgcn_cosf(u);
     }
     v = (t <= w.t_flat) ? 1.0f : v;
@@ -545,7 +549,9 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
     static_assert(!NW || (PREC == PREC_FP32 && !TAN && !ENS), "windowed normalization: FP32 forward");
     static_assert(!LBL || (MODE == MODE_PW && MT == 1 && !TAN && !ENS && !NW), "line scan: plane-wave forward, one tile");
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform: held in a scalar register, so the addresses of the
+    // wave's tile copies are formed on the scalar unit
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = lane >> 4, c = lane & 15;
     const long n_vox = (long)A.n_x * A.n_z;
     // line scan: block bz covers depths of grid column bz, the line of transmit row bz
@@ -804,14 +810,12 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             constexpr int cc = decltype(cc_const)::value;
             if (pipe) chunk_begin(cc_const, b);
             const float2 *xs = stage_chunk<MT, KI, cc>();
-            const int j_a = max(u_lo >> 2, cc * SJC), j_b = min(j_hi, cc * SJC + SJC - 1);
-#pragma clang loop unroll(disable)
-            for (int j = j_a; j <= j_b; ++j) {
+            // one element step: elements 4 j + g of this lane, against the chunk's tiles
+            auto step = [&](const int j) __attribute__((always_inline)) {
                 const float d = fmaf((float)j, dstep, dgb);
                 float a;
                 if (WCR == WC_COS_ALL) {
-                    float u = fabsf(d) * (d < 0.f ? ul : ur);  // revolutions
-                    u = u - rintf(u);
+                    const float u = __builtin_amdgcn_fractf(fabsf(d) * (d < 0.f ? ul : ur));  // revolutions
                     a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
                 } else if (WCR == WC_COS_RF) {
                     a = window_rf(fabsf(d), d < 0.f ? ql : qr, A.wrx.q_o);
@@ -835,7 +839,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     Lr = fmaf(-xv.y, bi, Lr);
                     Li = fmaf(xv.x, bi, Li);
                     Li = fmaf(xv.y, br, Li);
-                    continue;
+                    return;
                 }
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
@@ -851,6 +855,16 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                         T3[TAN ? mt : 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv.y, bp * tr, T3[TAN ? mt : 0], 0, 0, 0);
                     }
                 }
+                        };
+            const int j_a = max(u_lo >> 2, cc * SJC), j_b = min(j_hi, cc * SJC + SJC - 1);
+            if constexpr (WCR == WC_COS_ALL && !LBL && !TAN && !NW) {
+                // every element step (no support test): unrolled, so the phase words are
+                // read from fixed registers and the first step starts the accumulators
+#pragma unroll
+                for (int jj = 0; jj < SJC; ++jj) step(cc * SJC + jj);
+            } else {
+#pragma clang loop unroll(disable)
+                for (int j = j_a; j <= j_b; ++j) step(j);
             }
             if (pipe) chunk_end(cc_const, b);
             };  // chunk_step
@@ -901,8 +915,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     const float d = fmaf((float)(k - k0), pf, dgb);
                     float a;
                     if (WCR == WC_COS_ALL) {
-                        float u = fabsf(d) * (d < 0.f ? ul : ur);  // revolutions
-                        u = u - rintf(u);
+                        const float u = __builtin_amdgcn_fractf(fabsf(d) * (d < 0.f ? ul : ur));  // revolutions
                         a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
                     } else if (WCR == WC_COS_RF) {
                         a = window_rf(fabsf(d), d < 0.f ? ql : qr, A.wrx.q_o);
@@ -946,12 +959,22 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 }  // chunk in the support
                 if (pipe) chunk_end(cc_const, b);
             };  // chunk_lp
+            // every chunk of 32 elements: KC = 1, 2, 4, 6 or 8
             chunk_lp(std::integral_constant<int, 0>{});
             if constexpr (KC > 1) chunk_lp(std::integral_constant<int, 1>{});
             if constexpr (KC > 2) {
                 chunk_lp(std::integral_constant<int, 2>{});
                 chunk_lp(std::integral_constant<int, 3>{});
             }
+            if constexpr (KC > 4) {
+                chunk_lp(std::integral_constant<int, 4>{});
+                chunk_lp(std::integral_constant<int, 5>{});
+            }
+            if constexpr (KC > 6) {
+                chunk_lp(std::integral_constant<int, 6>{});
+                chunk_lp(std::integral_constant<int, 7>{});
+            }
+            static_assert(KC == 1 || KC == 2 || KC == 4 || KC == 6 || KC == 8, "chunks of 32 elements");
             (void)pl;
         }
         asum += __shfl_xor(asum, 16);
@@ -970,20 +993,21 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     const float yr = accR[0][0] * sc, yi = accI[0][0] * sc;
                     Ir += cs * yr - sn * yi;
                     Ii += cs * yi + sn * yr;
-                } else
+                } else {
+                float Sr = 0.f, Si = 0.f;  // sum over the rows of T Y, scaled once below
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
                         const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                        const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
                         if (ENS) {
+                            const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
                             Er[ENS ? mt : 0][r] += cs * yr - sn * yi;
                             Ei[ENS ? mt : 0][r] += cs * yi + sn * yr;
                         } else {
-                            Ir += cs * yr - sn * yi;
-                            Ii += cs * yi + sn * yr;
+                            Sr = fmaf(cs, accR[mt][r], fmaf(-sn, accI[mt][r], Sr));
+                            Si = fmaf(cs, accI[mt][r], fmaf(sn, accR[mt][r], Si));
                         }
                         if (TAN) {
                             const float tt = Ttx[TAN ? mt : 0][r];
@@ -993,6 +1017,11 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                             pi += cs * ui + sn * ur;
                         }
                     }
+                if (!ENS) {
+                    Ir = fmaf(sc, Sr, Ir);
+                    Ii = fmaf(sc, Si, Ii);
+                }
+                }
                 if (TAN) {
                     // dI += sc (j w_n) (-1 / c) P
                     const float f = -sc * (float)n * w_unit * inv_c;
@@ -1019,6 +1048,21 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     wtx[mt][r] = w;
                     tsum += w;
                 }
+            if (MT_all > MT) {
+                // transmit rows split over groups (arrays beyond 128 elements): the
+                // normalization sums the window over every transmitter, not only this
+                // group's (das_sa.m:479)
+                tsum = 0.f;
+                for (int t = 0; t < MT_all; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = 16 * t + 4 * g + r;
+                        const float d = fmaf((float)(16 * t + r), pf, dg_tx);
+                        const float tt = fabsf(d) * (d < 0.f ? at.inv_wl : at.inv_wr);
+                        const float w = window_eval<WCT>(tt, A.wtx);
+                        tsum += (i >= at.klo && i <= at.khi) ? w : 0.f;
+                    }
+            }
             tsum += __shfl_xor(tsum, 16);
             tsum += __shfl_xor(tsum, 32);
             if (NW && A.nw_tx) {
@@ -1095,13 +1139,10 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
     if (g == 0 && vok) {
         const float2 val = make_float2(A.out_scale * Ir, A.out_scale * Ii);
         float2 *dimg = reinterpret_cast<float2 *>(A.dimage);
-        if (A.atomic_out) {
-            atomicAdd(&A.image[v].x, val.x);
-            atomicAdd(&A.image[v].y, val.y);
-            if (TAN) {
-                atomicAdd(&dimg[v].x, A.out_scale * Dr);
-                atomicAdd(&dimg[v].y, A.out_scale * Di);
-            }
+        if (A.partial) {
+            const long slot = (long)by * A.slot_z + (LBL ? 0 : bz);
+            A.partial[slot * n_vox + v] = val;
+            if (TAN) A.partial[((long)A.n_slots + slot) * n_vox + v] = make_float2(A.out_scale * Dr, A.out_scale * Di);
         } else {
             A.image[v] = val;
             if (TAN) dimg[v] = make_float2(A.out_scale * Dr, A.out_scale * Di);
@@ -1158,6 +1199,32 @@ void beamform_persistent_kernel(const BeamArgs *__restrict__ jobs, const int4 *_
 }
 
 static __global__ void reset_counter_kernel(int *c) { *c = 0; }
+
+// sum of the partial images of every (bin chunk, transmit group) slot in slot
+// order: out[v] = sum_s P[s][v]; dout (tangent) from slots n_slots .. 2 n_slots - 1
+static __global__ __launch_bounds__(256) void reduce_partials_kernel(const float2 *__restrict__ P, int n_slots,
+                                                                     long n_vox, float2 *__restrict__ out,
+                                                                     float2 *__restrict__ dout)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_vox) return;
+    float2 a = P[v];
+    for (int k = 1; k < n_slots; ++k) {
+        const float2 q = P[(long)k * n_vox + v];
+        a.x += q.x;
+        a.y += q.y;
+    }
+    out[v] = a;
+    if (dout) {
+        float2 d = P[(long)n_slots * n_vox + v];
+        for (int k = 1; k < n_slots; ++k) {
+            const float2 q = P[(long)(n_slots + k) * n_vox + v];
+            d.x += q.x;
+            d.y += q.y;
+        }
+        dout[v] = d;
+    }
+}
 
 
 // -------------------------------------------------------------------------

@@ -282,19 +282,22 @@ class _Diff(C.Structure):
 def das_pw_diff(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0,
                 window=("tukey", 0.2), F=None, normalization="on", semantics="cpu",
                 spectra_mode=1, n_threads=0, norm_window=("boxcar", 0.0),
-                c_tof=None, delay_offsets=None, g=None, y=None, want_dc=False, virtual_sources=None):
+                c_tof=None, delay_offsets=None, g=None, y=None, want_dc=False, virtual_sources=None,
+                sel=None):
     """FP64 derivatives of the PW image (oracle.h or_das_pw_diff).
 
     g, y: complex [n_z, n_x] (image layout).  Returns a dict with 'image',
     'dIdc' (want_dc), 'vjp_delay' and 'vjp_c' (g given; vjp_c also needs
-    want_dc), 'adjoint' [n_tx, n_el, n_t] (y given)."""
+    want_dc), 'adjoint' [n_tx, n_el, n_t] (y given).
+    sel=(ix, iz): only those voxels; g and y are then [n_sel] in that order (the
+    cotangent is zero elsewhere) and 'image' / 'dIdc' are [n_sel]."""
     rf = np.asarray(rf)
     if rf.ndim == 2:
         rf = rf[None]
     n_tx, n_el, n_t = rf.shape
     angles = np.ascontiguousarray(np.atleast_1d(angles), dtype=np.float64)
     a, keep, re, im = _args(x, z, rf, f_s, width, pitch, c_0, f_bounds, index_t0, semantics,
-                            spectra_mode, n_threads, None)
+                            spectra_mode, n_threads, sel)
     a.n_t, a.n_el, a.n_tx = n_t, n_el, n_tx
     a.angles = _dp(angles)
     a.win_rx, a.win_rx_param = WIN[window[0]], float(window[1])
@@ -302,14 +305,15 @@ def das_pw_diff(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0
     a.norm_rx = NORM[normalization]
     a.norm_rx_win, a.norm_rx_win_param = WIN[norm_window[0]], float(norm_window[1])
     nx, nz = np.asarray(x).size, np.asarray(z).size
-    nv = nx * nz
+    nv = nx * nz if sel is None else np.asarray(sel[0]).size
     a.signal_re = a.signal_im = None
     _virtual_sources(a, keep, virtual_sources, n_tx)
     d = _Diff()
     out = {}
 
-    def vox(img):  # [n_z, n_x] -> oracle order (x-major, z fastest)
-        img = np.asarray(img, dtype=np.complex128).T.ravel()
+    def vox(img):  # [n_z, n_x] -> oracle order (x-major, z fastest); [n_sel] as given
+        img = np.asarray(img, dtype=np.complex128)
+        img = img.ravel() if sel is not None else img.T.ravel()
         r, i = np.ascontiguousarray(img.real), np.ascontiguousarray(img.imag)
         keep.extend([r, i])
         return r, i
@@ -334,9 +338,10 @@ def das_pw_diff(x, z, rf, f_s, angles, width, pitch, c_0, f_bounds, index_t0=0.0
                               _dp(offs) if offs is not None else None, C.byref(d))
     if rc:
         raise ValueError(f"or_das_pw_diff failed ({rc})")
-    out["image"] = (re + 1j * im).reshape(nx, nz).T
+    shape = (lambda v: v) if sel is not None else (lambda v: v.reshape(nx, nz).T)
+    out["image"] = shape(re + 1j * im)
     if want_dc:
-        out["dIdc"] = (dr + 1j * di).reshape(nx, nz).T
+        out["dIdc"] = shape(dr + 1j * di)
     if g is not None:
         out["vjp_delay"] = vj
         if want_dc:

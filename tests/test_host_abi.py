@@ -133,3 +133,23 @@ def test_synth_host_is_deterministic_and_focused():
     xs, zs, amp = w.scatterers()
     assert xs.size == w.n_points + w.n_speckle and np.all(amp[:w.n_points] == 10.0)
     assert np.all((xs >= -19e-3) & (xs <= 19e-3)) and np.all((zs >= 5e-3) & (zs <= 40e-3))
+
+
+@pytest.mark.parametrize("n_el,kw,needle", [
+    (300, {}, "at most 256 array elements"),
+    (200, {"n_frames": 2}, "ensembles: at most 128 elements"),
+    (160, {"line_scan": True}, "line scans: at most 128 elements"),
+])
+def test_wide_array_refusals_without_a_gpu(n_el, kw, needle):
+    """Arrays beyond 128 elements take the one-launch forward kernel of single-frame
+    plans (up to 256 elements); the other forms refuse them before any device work."""
+    px = np.linspace(-1e-3, 1e-3, 4)
+    extra = {}
+    if kw.get("line_scan"):
+        extra = dict(angles=np.zeros(px.size), virtual_sources=np.stack([px, np.full(px.size, 20e-3)], axis=1))
+    P, keep, info = _params(n_el=n_el, px=px, **extra, **kw)
+    L = _lib.load()
+    h = C.c_void_p()
+    st = L.fn_plan_pw_create(C.byref(P), C.byref(h))
+    assert L.fn_status_id(st).decode() == "Unsupported"
+    assert needle in L.fn_last_error().decode()

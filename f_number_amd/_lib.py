@@ -141,6 +141,8 @@ PROTOTYPES = {
     "fn_plan_vjp": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]),
     "fn_das_pw_adjoint": (C.c_int32, [C.POINTER(PwParams), C.c_double, _dp, _fp, _fp]),
+    "fn_das_pw_noalloc": (C.c_int32, [C.POINTER(PwParams), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
     "fn_synth_rf": (C.c_int32, [C.POINTER(SynthParams), C.c_void_p]),
     "fn_synth_scatterers": (C.c_int32, [C.POINTER(SynthParams), _dp, _dp, _dp]),
 }

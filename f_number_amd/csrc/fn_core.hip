@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1144,7 +1145,202 @@ int one_shot(fn_plan *P, const void *rf, int dtype, float *image, float *w0, flo
 
 }  // namespace
 
+// ---- the no-allocation entry point (fn_das_pw_noalloc) ---------------------
+// Every device table and the spectra workspace of one call live in `scratch`, a
+// file-scope array of this code object: each device holds its own copy at its
+// own address, which host code obtains once per device with hipGetSymbolAddress
+// and then uses exclusively (the array's name in host code is only a stand-in).
+// Calls on one device share the array and race if they overlap.
+namespace {
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace fnk {
+// host arrays reach the scratch as kernel arguments (captured at launch, so the
+// caller's arrays are free to change once the call returns): 448 doubles a launch
+struct UploadChunk {
+    double v[448];
+};
+
+__global__ void upload_chunk_kernel(double *dst, UploadChunk c, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = c.v[i];
+}
+
+// the tables pw_plan computes on the host, computed here from the raw inputs:
+// twiddles e^{+j 2 pi m / N}, sqrt(f_n) per bin, the steering rows
+// (das_pw.m:323-327: e_x = -sin, e_z = cos, x_ref = sign(e_x) x_0; or the
+// virtual sources (x_vs, z_vs, |r_vs|)) and zero delay offsets
+__global__ void noalloc_tables_kernel(float2 *tw, int N, double *sqf, int n_f, int n_lb, double f_s, double *txp,
+                                      const double *raw_tx, int n_tx, int virt, double M, double pitch, double *delay,
+                                      int n_el)
+{
+    const int n = max(max(N, n_f), max(n_tx, n_el));
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (i < N) {
+            double sn, cs;
+            sincospi(2.0 * (double)i / (double)N, &sn, &cs);
+            tw[i] = make_float2((float)cs, (float)sn);
+        }
+        if (i < n_f) sqf[i] = sqrt(f_s * (double)(n_lb + i) / N);
+        if (i < n_tx) {
+            if (virt) {
+                const double xv = raw_tx[2 * i], zv = raw_tx[2 * i + 1];
+                txp[3 * i] = xv;
+                txp[3 * i + 1] = zv;
+                txp[3 * i + 2] = sqrt(xv * xv + zv * zv);
+            } else {
+                const double th = raw_tx[i];
+                const double ex = -sin(th), ez = cos(th);
+                const double sg = (ex > 0) - (ex < 0);
+                txp[3 * i] = ex;
+                txp[3 * i + 1] = ez;
+                txp[3 * i + 2] = sg * ((0 - M) * pitch);
+            }
+        }
+        if (i < n_el) delay[i] = 0.0;
+    }
+}
+}  // namespace fnk
+
+namespace {
+
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on `device`, looked up on the first call on that
+// device (which is current: select_device ran) and kept
+int scratch_on_device(int device, unsigned char **out)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(FN_ERR_DEVICE, "device index %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(FN_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = static_cast<unsigned char *>(g_scratch_ptr[device]);
+    return FN_OK;
+}
+
+int upload_chunked(double *dst, const double *src, long n, hipStream_t s)
+{
+    fnk::UploadChunk c;
+    for (long i0 = 0; i0 < n; i0 += 448) {
+        const int m = (int)std::min<long>(448, n - i0);
+        std::memcpy(c.v, src + i0, (size_t)m * sizeof(double));
+        hipLaunchKernelGGL(fnk::upload_chunk_kernel, dim3(1), dim3(256), 0, s, dst + i0, c, m);
+        HIP_CHECK(hipGetLastError(), "upload_chunk_kernel");
+    }
+    return FN_OK;
+}
+
+// sub-allocation of the scratch: 256-byte aligned pieces
+struct Carve {
+    size_t used = 0;
+    size_t take(size_t bytes)
+    {
+        const size_t at = used;
+        used += (bytes + 255) & ~(size_t)255;
+        return at;
+    }
+};
+
+int das_pw_noalloc(const fn_pw_params *p, const void *d_rf, int dtype, float *d_image, float *d_w, hipStream_t s)
+{
+    fnh::Grid g;
+    int st = fnh::validate_pw(p, &g);
+    if (st) return st;
+    if (!d_rf) return fail(FN_ERR_RF_DATA, "RF data missing");
+    if (!d_image) return fail(FN_ERR_ARGUMENT, "image buffer missing");
+    if (dtype != FN_DTYPE_F32 && dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
+    if (p->n_frames > 1) return fail(FN_ERR_UNSUPPORTED, "fn_das_pw_noalloc: one frame per call (use a plan for ensembles)");
+    if (p->line_scan) return fail(FN_ERR_UNSUPPORTED, "fn_das_pw_noalloc: line scans need a plan");
+    if (p->normalization == FN_NORMALIZATION_WINDOWED && p->semantics == FN_SEMANTICS_CPU)
+        return fail(FN_ERR_UNSUPPORTED, "fn_das_pw_noalloc: normalizations.windowed (CPU semantics) needs a plan");
+    // the configuration of fn_plan_pw_create on a plan held on the stack: the same
+    // tiling and kernels, with every device pointer in the scratch
+    fn_plan P;
+    Common c{};
+    if ((st = configure_pw(&P, p, &c))) return st;
+    if ((st = configure_common(&P, c, g))) return st;
+    use_cos_all(&P);
+    const int n_f = g.n_f, n_tx = c.n_tx, raw_per_tx = p->virtual_sources ? 2 : 1;
+    Carve cv;
+    const size_t o_X = cv.take(P.x_bytes);
+    const size_t o_Xlp = cv.take(P.xlp_bytes);
+    const size_t o_xs = cv.take(P.precision != FN_PRECISION_FP32 ? (size_t)n_f * sizeof(float) : 0);
+    const size_t o_px = cv.take((size_t)c.n_x * sizeof(double));
+    const size_t o_pz = cv.take((size_t)c.n_z * sizeof(double));
+    const size_t o_F = cv.take((size_t)n_f * sizeof(double));
+    const size_t o_sqf = cv.take((size_t)n_f * sizeof(double));
+    const size_t o_raw = cv.take((size_t)n_tx * raw_per_tx * sizeof(double));
+    const size_t o_txp = cv.take((size_t)n_tx * 3 * sizeof(double));
+    const size_t o_tw = cv.take((size_t)g.N * sizeof(float2));
+    const size_t o_dl = cv.take((size_t)c.n_el * sizeof(double));
+    const int slots = beam_slots(&P);
+    const size_t o_pi = cv.take(slots > 1 ? (size_t)slots * P.n_vox * sizeof(float2) : 0);
+    if (cv.used > kScratchBytes)
+        return fail(FN_ERR_UNSUPPORTED,
+                    "fn_das_pw_noalloc: this call needs %zu bytes of device scratch, the entry point holds %zu "
+                    "(spectra workspace %zu bytes); use a plan",
+                    cv.used, kScratchBytes, P.x_bytes + P.xlp_bytes);
+    if ((st = select_device(p->index_gpu))) return st;
+    unsigned char *base = nullptr;
+    if ((st = scratch_on_device(p->index_gpu, &base))) return st;
+    P.d_X = reinterpret_cast<float2 *>(base + o_X);
+    P.d_Xlp = P.xlp_bytes ? base + o_Xlp : nullptr;
+    P.d_xscale = P.xlp_bytes ? reinterpret_cast<float *>(base + o_xs) : nullptr;
+    P.d_px = reinterpret_cast<double *>(base + o_px);
+    P.d_pz = reinterpret_cast<double *>(base + o_pz);
+    P.d_Frx = reinterpret_cast<double *>(base + o_F);
+    P.d_sqf = reinterpret_cast<double *>(base + o_sqf);
+    P.d_txp = reinterpret_cast<double *>(base + o_txp);
+    P.d_tw = reinterpret_cast<float2 *>(base + o_tw);
+    P.d_delay = reinterpret_cast<double *>(base + o_dl);
+    P.d_pimg = slots > 1 ? reinterpret_cast<float2 *>(base + o_pi) : nullptr;
+    double *raw = reinterpret_cast<double *>(base + o_raw);
+    if ((st = upload_chunked(P.d_px, p->positions_x, c.n_x, s))) return st;
+    if ((st = upload_chunked(P.d_pz, p->positions_z, c.n_z, s))) return st;
+    if ((st = upload_chunked(P.d_Frx, p->f_number_values, n_f, s))) return st;
+    if ((st = upload_chunked(raw, p->virtual_sources ? p->virtual_sources : p->steering_angles, (long)n_tx * raw_per_tx,
+                             s)))
+        return st;
+    const int n_tab = std::max(std::max(g.N, n_f), std::max(n_tx, c.n_el));
+    hipLaunchKernelGGL(fnk::noalloc_tables_kernel, dim3((unsigned)((n_tab + 255) / 256)), dim3(256), 0, s, P.d_tw,
+                       g.N, P.d_sqf, n_f, g.n_lb, c.f_s, P.d_txp, raw, n_tx, p->virtual_sources ? 1 : 0,
+                       (p->n_elements - 1) / 2.0, p->element_pitch, P.d_delay, c.n_el);
+    HIP_CHECK(hipGetLastError(), "noalloc_tables_kernel");
+    fnk::BeamArgs &A = P.args;
+    A.X = P.d_X;
+    A.Xlp = P.d_Xlp;
+    A.xscale = P.d_xscale;
+    A.px = P.d_px; A.pz = P.d_pz; A.Frx = P.d_Frx; A.Ftx = nullptr; A.txp = P.d_txp; A.sqrt_f = P.d_sqf;
+    A.delay = P.d_delay;
+    // spectra (every operand slot of the workspace is written, padding as zeros) and
+    // the 16-bit planes, the beamformer, and the sum of the partial images
+    if ((st = prepare(&P, d_rf, dtype, d_image, s))) return st;
+    const fnk::BeamArgs B = beam_args(&P, d_image, d_w, nullptr);
+    hipError_t e = launch_beam(&P, dim3((unsigned)beam_tiles(&P), (unsigned)P.bin_split, (unsigned)P.n_groups), B, s);
+    if (e != hipSuccess) return hip_fail(e, "beamform_kernel");
+    return finish_beam(&P, B, s);
+}
+
+}  // namespace
+
 extern "C" {
+
+int32_t fn_das_pw_noalloc(const fn_pw_params *p, const void *d_rf, int32_t rf_dtype, float *d_image, float *d_weights,
+                          void *stream)
+{
+    fnh::clear_error();
+    return das_pw_noalloc(p, d_rf, rf_dtype, d_image, d_weights, static_cast<hipStream_t>(stream));
+}
 
 int32_t fn_device_count(int32_t *count)
 {

@@ -216,6 +216,26 @@ int32_t fn_das_pw(const fn_pw_params *p, const void *data_rf, int32_t rf_dtype, 
 int32_t fn_das_sa(const fn_sa_params *p, const void *data_rf, int32_t rf_dtype, float *image,
                   float *weights_tx, float *weights_rx);
 
+/* No counterpart in the reference: an entry point that allocates nothing, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Plane waves, one
+ * frame: fn_das_pw's computation on device buffers -- d_rf (rf_dtype) in the
+ * layout of fn_plan_execute, d_image complex float [n_x][n_z], d_weights (may
+ * be NULL) float [n_x][n_z] -- enqueued on `stream` (NULL: the default stream)
+ * without host synchronisation.  The plan tables and the spectra workspace live
+ * in a fixed 32 MiB device array of the library (one per GPU, held until the
+ * process ends); a call that needs more returns FN_ERR_UNSUPPORTED before any
+ * device work (the FP32 spectra take 16 KiB per band bin and group of 16 angles
+ * at 128 elements, so about 2000 (bin, angle group) pairs fit: 11 angles over
+ * 542 bins take 8.9 MB, 75 angles over 1455 bins would take 119 MB; the 16-bit
+ * forms need twice that per pair).  Ensembles, line scans and windowed normalization (CPU
+ * semantics) also return FN_ERR_UNSUPPORTED.  The host arrays of p are read
+ * before the call returns.  Every call on one GPU uses that GPU's array, so
+ * calls that overlap on one GPU (other host threads, or other streams still
+ * running an earlier call) race. */
+int32_t fn_das_pw_noalloc(const fn_pw_params *p, const void *d_rf, int32_t rf_dtype, float *d_image,
+                          float *d_weights, void *stream);
+
 /* ---- plans: device-resident execution ---------------------------------- */
 /* A plan holds the validated configuration, the device copies of the grid and
  * F-number tables and the spectra workspace for one shape.  fn_plan_execute

@@ -153,3 +153,19 @@ def test_wide_array_refusals_without_a_gpu(n_el, kw, needle):
     st = L.fn_plan_pw_create(C.byref(P), C.byref(h))
     assert L.fn_status_id(st).decode() == "Unsupported"
     assert needle in L.fn_last_error().decode()
+
+
+def test_noalloc_refuses_beyond_scratch_without_a_gpu():
+    """fn_das_pw_noalloc: 75 angles over 1455 bins need 119 MB of spectra workspace,
+    beyond its 32 MiB device array: refused before any device work."""
+    from f_number_amd import synth
+    from f_number_amd.das import pw_params
+
+    w = synth.cpwc_full()
+    P, keep, info = pw_params(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width,
+                              w.pitch, w.c_0, w.f_bounds, 0.0, fa.windows.hann(), fa.f_numbers.grating.angle_lb(60, 1.5),
+                              fa.normalizations.on())
+    L = _lib.load()
+    st = L.fn_das_pw_noalloc(C.byref(P), C.c_void_p(16), 0, C.c_void_p(16), None, None)
+    assert L.fn_status_id(st).decode() == "Unsupported"
+    assert "scratch" in L.fn_last_error().decode()

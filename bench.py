@@ -82,6 +82,8 @@ def parse():
     ap.add_argument("--no-batching", action="store_true", help="skip the batching-form measurements")
     ap.add_argument("--no-saft", action="store_true", help="skip the SAFT precision-form measurements")
     ap.add_argument("--no-line-scan", action="store_true", help="skip the line-by-line scanning measurement")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 11-angle 256 x 256 CPWC arm (the CPU-runnable workload at GPU scale)")
     ap.add_argument("--bin-split", type=int, default=0,
                     help="bin split of the headline arm's plan (0: the plan's default tiling)")
     ap.add_argument("--ensemble-frames", type=int, default=400)
@@ -322,6 +324,40 @@ def _time_ms(fn, steps, warmup, dev):
     return e0.elapsed_time(e1) / steps
 
 
+def run_small(steps, warmup, dev):
+    """The 11-angle CPWC workload of the CPU-runnable case at GPU scale: 128 elements,
+    -10..10 deg, 2048 samples, 2.25-6.75 MHz, 256 x 256 voxels, grating.angle_lb(60, 1.5)
+    with Tukey(0.5), normalization on; the default tiling and the autotuned one."""
+    import torch
+
+    import f_number_amd as fa
+    from f_number_amd import synth
+
+    w = synth.cpwc_small()
+    kw = dict(window=fa.windows.tukey(0.5), F_number=fa.f_numbers.grating.angle_lb(60, 1.5))
+    r = run_arm(w, kw, max(steps, 10), warmup, dev)
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     w.c_0, w.f_bounds, 0.0, index_gpu=dev.index or 0, **kw)
+    rf = w.rf_device(dev)
+    img = torch.empty((w.positions_x.size, w.positions_z.size), dtype=torch.complex64, device=dev)
+    tune = plan.autotune(rf, img, repetitions=3)
+    plan.set_profiling(True)
+    plan.get_profile()
+    for _ in range(max(steps, 10)):
+        plan.execute(rf, img)
+    torch.cuda.synchronize(dev)
+    _, beam_tuned, n = plan.get_profile()
+    plan.close()
+    beam_tuned /= max(n, 1)
+    flops = r["info"]["flops_per_frame"]
+    return {"workload": w.name, "n_voxels": int(w.n_voxels), "value": w.n_voxels / (r["ms_per_step"] / 1e3) / 1e6,
+            "unit": "Mvoxel*frames/s", "ms_per_step": r["ms_per_step"], "spectra_ms": r["spectra_ms"],
+            "beamform_ms": r["beamform_ms"], "roofline": precision_roofline("fp32", flops, r["beamform_ms"]),
+            "tiling": {"m_tiles": r["info"]["m_tiles"], "bin_split": r["info"]["bin_split"]},
+            "autotuned": {"m_tiles": tune["m_tiles"], "bin_split": tune["bin_split"], "beamform_ms": beam_tuned,
+                          "roofline": precision_roofline("fp32", flops, beam_tuned)}}
+
+
 def run_diff(steps, warmup, dev):
     """Differentiation form on the layered-medium acquisition (11 angles, 2048 samples,
     384 x 384): forward, adjoint, tangent and vector-Jacobian timings (all enqueued on
@@ -354,6 +390,9 @@ def run_diff(steps, warmup, dev):
     t_tan = _time_ms(lambda: plan.execute_tangent(rf, img, dimg), steps, warmup, dev)
     g = (2 * img).contiguous()
     t_vjp = _time_ms(lambda: plan.vjp(rf, g, dimg), steps, warmup, dev)
+    # the forward at its autotuned tiling, against which the adjoint is also compared
+    tune = plan.autotune(rf, img, repetitions=3)
+    t_fwd_tuned = _time_ms(lambda: plan.execute(rf, img), steps, warmup, dev)
     res = AF.autofocus(plan, rf, n_iter=50, c0=1480.0)
     plan.close()
     truth = w.delay_offsets - w.delay_offsets.mean()
@@ -364,6 +403,9 @@ def run_diff(steps, warmup, dev):
         "forward_ms": t_fwd, "adjoint_ms": t_adj, "adjoint_first_call_ms": t_first, "tangent_ms": t_tan,
         "vjp_ms": t_vjp,
         "forward_plus_adjoint_throughput_vs_forward": t_fwd / (t_fwd + t_adj),
+        "forward_autotuned_ms": t_fwd_tuned, "autotuned_tiling": {"m_tiles": tune["m_tiles"],
+                                                                 "bin_split": tune["bin_split"]},
+        "forward_plus_adjoint_throughput_vs_autotuned_forward": t_fwd_tuned / (t_fwd_tuned + t_adj),
         "autofocus": {"iterations": 50, "seconds": res["seconds"], "c_start": 1480.0, "c_final": res["c"],
                       "delay_rms_error_ns_start": float(np.std(truth) * 1e9),
                       "delay_rms_error_ns_final": float(np.std(d - truth) * 1e9),
@@ -617,6 +659,8 @@ def main():
                  for k, v in arms.items()},
         "precision_arms": prec_arms,
     }
+    if not args.no_small:
+        out["cpwc_11ang_256x256"] = run_small(args.steps, args.warmup, dev)
     if not args.no_diff:
         out["differentiation"] = run_diff(max(3, args.steps // 2), 1, dev)
     if not args.no_line_scan:

@@ -127,9 +127,13 @@ __device__ __forceinline__ uint32_t frac32(double q)
     return (uint32_t)(int64_t)rint(f * 4294967296.0);
 }
 
+// The phase word P (units of 2^-32 revolutions) as the float 1 + P 2^-32 with 23
+// fraction bits, in one instruction: bits 0x3F800000 | (P >> 9).  v_sin_f32 /
+// v_cos_f32 take revolutions and drop the integer part.  Truncating the low 9 bits
+// errs by at most 2^-23 revolutions (7.5e-7 rad).
 __device__ __forceinline__ float rev_of(uint32_t P)
 {
-    return (float)(int32_t)P * 2.3283064365386963e-10f;  // 2^-32
+    return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, P, 9u));
 }
 
 // Per (voxel, bin) aperture: das_pw.m:387, 399-411 (CPU) and
@@ -805,6 +809,10 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             const float dstep = 4.0f * pf;
             const float ul = ap.inv_wl * A.wrx.s_half, ur = ap.inv_wr * A.wrx.s_half;
             const float ql = ap.inv_wl * A.wrx.q_s, qr = ap.inv_wr * A.wrx.q_s;  // WC_COS_RF slopes
+            // WC_COS_ALL: the cosine argument |d| / (2 W_side) of element step j without a
+            // per-element side select: max(d ur, -d ul) with d = dgb + j dstep (each term is
+            // positive exactly on its own side)
+            const float ar0 = dgb * ur, ar1 = dstep * ur, al0 = -dgb * ul, al1 = -dstep * ul;
             const int j_hi = min(u_hi >> 2, KI - 1);  // elements 4j + g, g = 0..3
             auto chunk_step = [&](auto cc_const) {
             constexpr int cc = decltype(cc_const)::value;
@@ -815,7 +823,8 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 const float d = fmaf((float)j, dstep, dgb);
                 float a;
                 if (WCR == WC_COS_ALL) {
-                    const float u = __builtin_amdgcn_fractf(fabsf(d) * (d < 0.f ? ul : ur));  // revolutions
+                    const float jf = (float)j;
+                    const float u = __builtin_amdgcn_fractf(fmaxf(fmaf(jf, ar1, ar0), fmaf(jf, al1, al0)));
                     a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
                 } else if (WCR == WC_COS_RF) {
                     a = window_rf(fabsf(d), d < 0.f ? ql : qr, A.wrx.q_o);
@@ -900,6 +909,8 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             }
             const float ul = ap.inv_wl * A.wrx.s_half, ur = ap.inv_wr * A.wrx.s_half;
             const float ql = ap.inv_wl * A.wrx.q_s, qr = ap.inv_wr * A.wrx.q_s;  // WC_COS_RF slopes
+            // WC_COS_ALL: |d| / (2 W_side) = max(d ur, -d ul), d = dgb + (k - k0) pitch
+            const float ar0 = dgb * ur, ar1 = pf * ur, al0 = -dgb * ul, al1 = -pf * ul;
             auto chunk_lp = [&](auto cc_const) {
                 constexpr int cc = decltype(cc_const)::value;
                 if (pipe) chunk_begin(cc_const, b);
@@ -915,7 +926,8 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     const float d = fmaf((float)(k - k0), pf, dgb);
                     float a;
                     if (WCR == WC_COS_ALL) {
-                        const float u = __builtin_amdgcn_fractf(fabsf(d) * (d < 0.f ? ul : ur));  // revolutions
+                        const float e = (float)(k - k0);
+                        const float u = __builtin_amdgcn_fractf(fmaxf(fmaf(e, ar1, ar0), fmaf(e, al1, al0)));
                         a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
                     } else if (WCR == WC_COS_RF) {
                         a = window_rf(fabsf(d), d < 0.f ? ql : qr, A.wrx.q_o);

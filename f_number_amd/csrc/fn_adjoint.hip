@@ -38,7 +38,8 @@ namespace fnk {
 
 constexpr int ADJ_WAVES = 4;   // bins per block, one per wave (two blocks per CU)
 constexpr int ADJ_ET = 8;      // element tiles of 16 (128 elements)
-constexpr int ADJ_QS = 132;    // LDS words per voxel of the receive phase table ([cidx][mt] + pad)
+constexpr int ADJ_QS = 144;    // LDS words per voxel of the receive phase table ([mt][cidx] + pad: lanes
+                               // 0-31 of a read meet 32 distinct banks)
 constexpr int ADJ_GV = 32;     // voxels per LDS group (one barrier each): two 16-voxel contraction groups
 
 // FULL: no per-element support test on a 128-element array (no padded
@@ -52,7 +53,7 @@ template <int WCR, bool FULL>
 __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(AdjArgs S)
 {
     // group tables, double-buffered: the next group is loaded while this one is contracted
-    __shared__ __align__(16) uint32_t q_rx[2][ADJ_GV * ADJ_QS];  // [voxel i][cidx][mt]: element cidx + 16 mt
+    __shared__ __align__(16) uint32_t q_rx[2][ADJ_GV * ADJ_QS];  // [voxel i][mt][cidx]: element cidx + 16 mt
     __shared__ uint32_t q_tx[2][ADJ_GV * 16];      // [voxel i][angle column]
     __shared__ float2 vy[2][ADJ_GV];               // cotangent of the group's voxels (0 outside the range)
     __shared__ float scv[2][ADJ_WAVES][ADJ_GV];    // s / w_n(v) of each wave's bin for the group's voxels
@@ -186,13 +187,25 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
 
             // Gauss: conj(B) U = (br - j bi)(ur + j ui):
             //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = bi (ur + ui); Re = K1 + K3, Im = K1 + K2
+            // Per voxel step s (voxel 4 s + q of the group): U, and the window argument as
+            // max(d r1 + r0, d l1 + l0) over the element tile mt, d = x_k - x = dg + 16 mt pitch
+            // (each side's term is the larger exactly on its own side, so no per-element select)
+            float Ur[4], U2[4], U3[4], R0[4], R1[4], L0[4], L1[4];
+            int Klo[4], Khi[4];
+            int qoff[4];
     #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int i = 4 * s + q;  // voxel of this lane in step s
                 const uint4 apv = apx[wave][16 * gq + i];  // written by this wave (LDS operations of a wave are ordered)
-                const int klo = (int)(apv.x & 0xffffu), khi = (int)(apv.x >> 16);
+                Klo[s] = (int)(apv.x & 0xffffu);
+                Khi[s] = (int)(apv.x >> 16);
                 const float iwl = __uint_as_float(apv.y), iwr = __uint_as_float(apv.z);
                 const float dg = fmaf((float)cidx, pf, __uint_as_float(apv.w));  // x_cidx - x
+                const float qo = WCR == WC_COS_RF ? A.wrx.q_o : 0.f;
+                R0[s] = fmaf(dg, iwr, qo);
+                R1[s] = 16.f * pf * iwr;
+                L0[s] = fmaf(-dg, iwl, qo);
+                L1[s] = -16.f * pf * iwl;
                 // s / w_n(v) of voxel i (0 outside a valid aperture), per-plan table
                 const float sc = a_ok ? scv[cur][wave][o + i] : 0.f;
                 // U = s y conj(T) / w
@@ -201,29 +214,35 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
                 const float st = __builtin_amdgcn_sinf(rt), ct = __builtin_amdgcn_cosf(rt);
                 const float ur = sc * (yv.x * ct + yv.y * st);
                 const float ui = sc * (yv.y * ct - yv.x * st);
-                const float u2 = ui - ur, u3 = ur + ui;
-                const uint4 qa = *reinterpret_cast<const uint4 *>(&q_rx[cur][(o + i) * ADJ_QS + cidx * ADJ_ET]);
-                const uint4 qb = *reinterpret_cast<const uint4 *>(&q_rx[cur][(o + i) * ADJ_QS + cidx * ADJ_ET + 4]);
-                const uint32_t Q[ADJ_ET] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+                Ur[s] = ur;
+                U2[s] = ui - ur;
+                U3[s] = ur + ui;
+                qoff[s] = (o + i) * ADJ_QS + cidx;
+            }
+            // element tile outer, voxel step inner: four independent chains per tile
     #pragma unroll
-                for (int mt = 0; mt < ADJ_ET; ++mt) {
-                    if (mt < t_lo || mt > t_hi) continue;
-                    const int e = cidx + 16 * mt;
-                    const float d = fmaf((float)(16 * mt), pf, dg);
+            for (int mt = 0; mt < ADJ_ET; ++mt) {
+                if (mt < t_lo || mt > t_hi) continue;
+                const int e = cidx + 16 * mt;
+    #pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const uint32_t Qv = q_rx[cur][qoff[s] + 16 * mt];
+                    const float ta = mt == 0 ? R0[s] : fmaf((float)mt, R1[s], R0[s]);
+                    const float tb = mt == 0 ? L0[s] : fmaf((float)mt, L1[s], L0[s]);
                     float w;
                     if (WCR == WC_COS_RF) {
-                        w = window_rf(fabsf(d), d < 0.f ? iwl : iwr, A.wrx.q_o);
+                        const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(fmaxf(ta, tb), 0.f, 0.25f));
+                        w = c * c;
                     } else {
-                        const float t = fabsf(d) * (d < 0.f ? iwl : iwr);
-                        w = window_eval<WCR>(t, A.wrx);
+                        w = window_eval<WCR>(fmaxf(ta, tb), A.wrx);
                     }
-                    if (!FULL) w = (e >= klo && e <= khi) ? w : 0.f;
-                    const float rv = rev_of(n * Q[mt]);
+                    if (!FULL) w = (e >= Klo[s] && e <= Khi[s]) ? w : 0.f;
+                    const float rv = rev_of(n * Qv);
                     const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
                     const float br = w * cs, bi = w * sn;
-                    K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br - bi, ur, K1[mt], 0, 0, 0);
-                    K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, u2, K2[mt], 0, 0, 0);
-                    K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi, u3, K3[mt], 0, 0, 0);
+                    K1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br - bi, Ur[s], K1[mt], 0, 0, 0);
+                    K2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(br, U2[s], K2[mt], 0, 0, 0);
+                    K3[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi, U3[s], K3[mt], 0, 0, 0);
                 }
             }
         }  // active
@@ -249,8 +268,8 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
 
 // Phase words of every voxel for the voxel contraction: FP64 time of flight
 // (das_pw.m:336, 339-340, 427-428 with c_tof and the delay offsets) reduced to
-// frac(f_s tau / N) in units of 2^-32.  qrx[v][cidx][mt] holds element
-// cidx + 16 mt (zero-padded to 128 elements), qtx[v][a] angle a (16 n_at columns).
+// frac(f_s tau / N) in units of 2^-32.  qrx[v][e] holds element e (zero-padded
+// to 128 elements), qtx[v][a] angle a (16 n_at columns).
 __global__ __launch_bounds__(256) void adjoint_tables_kernel(BeamArgs A, uint32_t *__restrict__ qrx,
                                                              uint32_t *__restrict__ qtx, double *__restrict__ vgeo,
                                                              int n_at)
@@ -273,7 +292,7 @@ __global__ __launch_bounds__(256) void adjoint_tables_kernel(BeamArgs A, uint32_
         const int e = col;
         const double lat = (((double)e - A.M) * A.pitch - x) / A.ctof, zc = z / A.ctof;
         const double tau = sqrt(lat * lat + zc * zc) + (e < A.n_el ? A.delay[e] : 0.0);
-        qrx[v * 128 + (e & 15) * 8 + (e >> 4)] = frac32(A.f_s * tau / A.N);
+        qrx[v * 128 + e] = frac32(A.f_s * tau / A.N);
     } else {
         const int a = col - 128;
         double tau = 0.0;

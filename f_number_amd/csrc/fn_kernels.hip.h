@@ -92,6 +92,8 @@ struct BeamArgs {
     int slot_z, n_slots;
     double M, pitch, width, c0, f_s, t0;
     int rows_per_frame;     // ensemble: transmit rows per frame (angles); images [frame][n_vox]
+    const int *row_frame;   // row stack: frame of every transmit row (rows_per_frame unused)
+    float2 *const *frame_image;  // row stack: image of every frame
     int tx_kind;            // TX_PLANE: txp = (e_x, e_z, x_ref); TX_VIRTUAL: txp = (x_vs, z_vs, |r_vs|)
     double ctof;            // speed of sound of the time of flight (c0 unless set)
     const double *delay;    // per-element delay offsets (s), [n_el]
@@ -109,7 +111,7 @@ struct AdjArgs {
     BeamArgs B;
     const float2 *y;        // cotangent image [n_vox]
     float *P;               // partials [n_vr][n_f][n_at][2][128][16]
-    const uint32_t *qrx;    // receive phase words [n_vox][cidx 16][mt 8] (element cidx + 16 mt)
+    const uint32_t *qrx;    // receive phase words [n_vox][element 128]
     const uint32_t *qtx;    // transmit phase words [n_vox][16 n_at]
     const double *vgeo;     // per voxel: x, x / pitch, z / (2 pitch), sqrt(z / (2.88 c0))
     const float *vsc;       // per bin and voxel [n_f][n_vox]: s / w_n(v), 0 outside a valid aperture
@@ -119,6 +121,10 @@ struct AdjArgs {
 
 struct SpecArgs {
     int n_t, n_ch, N, n_lb, n_f, n_el, n_tx, MT, KI;
+    // output rows: row a of this RF goes to row row0 + a of an operand layout of MT_out
+    // tiles (a row stack of several acquisitions); write_pad also writes the zero rows
+    // past n_tx of the last tile (a plan of its own)
+    int row0, MT_out, write_pad;
 };
 
 __device__ __forceinline__ uint32_t frac32(double q)
@@ -369,11 +375,13 @@ __global__ __launch_bounds__(256) void spectra_kernel(const T *__restrict__ rf, 
         }
     }
     // lane holds D[bin 16 wave + 4 q + r][column r16] of N-tile j
+    const int a_row = 16 * A + r16, g_row = S.row0 + a_row;  // row in this RF, row in the output
+    const bool put = a_row < S.n_tx || S.write_pad;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int b = b0 + 16 * wave + 4 * q + r;
-        if (b >= S.n_f) continue;
-        float2 *dst = X + (((size_t)b * S.MT + A) * S.KI + K4) * 64 + r16;
+        if (b >= S.n_f || !put) continue;
+        float2 *dst = X + (((size_t)b * S.MT_out + (g_row >> 4)) * S.KI + K4) * 64 + (g_row & 15);
 #pragma unroll
         for (int j = 0; j < 4; ++j) dst[16 * j] = make_float2(accR[j][r], accI[j][r]);
     }
@@ -538,7 +546,8 @@ struct PersistWavesPerEU {
 // a second contraction with B' = B tau_rx,g and the row factor tau_tx,g.
 // ENS (plane waves): the transmit rows are (frame, angle) pairs of an ensemble
 // sharing one geometry; every row keeps its own image accumulator over the
-// bins and is added to its frame's image at the end (frame = row / rows_per_frame).
+// bins and is added to its frame's image at the end (frame = row / rows_per_frame,
+// or row_frame[row] for a row stack of acquisitions with their own images).
 // NW: windowed normalization (FP32, one bin chunk per block so that every lane
 // counts its voxel's valid rows from the first bin).
 // One workgroup's share of a plan: voxel tile bx (64 voxels), bin chunk by,
@@ -813,6 +822,11 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             // per-element side select: max(d ur, -d ul) with d = dgb + j dstep (each term is
             // positive exactly on its own side)
             const float ar0 = dgb * ur, ar1 = dstep * ur, al0 = -dgb * ul, al1 = -dstep * ul;
+            // the other classes the same way: |d| / W_side = max(d / W_R, -d / W_L), and for
+            // WC_COS_RF the cosine argument |d| q_s / W_side + q_o
+            const float qo = WCR == WC_COS_RF ? A.wrx.q_o : 0.f;
+            const float sr = WCR == WC_COS_RF ? qr : ap.inv_wr, sl = WCR == WC_COS_RF ? ql : ap.inv_wl;
+            const float tr0 = fmaf(dgb, sr, qo), tr1 = dstep * sr, tl0 = fmaf(-dgb, sl, qo), tl1 = -dstep * sl;
             const int j_hi = min(u_hi >> 2, KI - 1);  // elements 4j + g, g = 0..3
             auto chunk_step = [&](auto cc_const) {
             constexpr int cc = decltype(cc_const)::value;
@@ -820,18 +834,23 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             const float2 *xs = stage_chunk<MT, KI, cc>();
             // one element step: elements 4 j + g of this lane, against the chunk's tiles
             auto step = [&](const int j) __attribute__((always_inline)) {
-                const float d = fmaf((float)j, dstep, dgb);
+                const float jf = (float)j;
                 float a;
                 if (WCR == WC_COS_ALL) {
-                    const float jf = (float)j;
                     const float u = __builtin_amdgcn_fractf(fmaxf(fmaf(jf, ar1, ar0), fmaf(jf, al1, al0)));
                     a = fmaf(A.wrx.B, __builtin_amdgcn_cosf(u), A.wrx.A);
-                } else if (WCR == WC_COS_RF) {
-                    a = window_rf(fabsf(d), d < 0.f ? ql : qr, A.wrx.q_o);
                 } else {
-                    const float t = fabsf(d) * (d < 0.f ? ap.inv_wl : ap.inv_wr);
-                    a = window_eval<WCR>(t, A.wrx);
-                    if (RTEST) a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
+                    // (wide arrays keep the per-element side select: fewer live registers)
+                    const float d = fmaf(jf, dstep, dgb);
+                    const float t = KI <= 32 ? fmaxf(fmaf(jf, tr1, tr0), fmaf(jf, tl1, tl0))
+                                             : fmaf(fabsf(d), d < 0.f ? sl : sr, qo);
+                    if (WCR == WC_COS_RF) {
+                        const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(t, 0.f, 0.25f));
+                        a = c * c;
+                    } else {
+                        a = window_eval<WCR>(t, A.wrx);
+                        if (RTEST) a = (4 * j >= klo && 4 * j <= khi) ? a : 0.f;
+                    }
                 }
                 const float rv = rev_of((uint32_t)n * Qrx[j]);
                 const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
@@ -871,9 +890,25 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 // read from fixed registers and the first step starts the accumulators
 #pragma unroll
                 for (int jj = 0; jj < SJC; ++jj) step(cc * SJC + jj);
-            } else {
+            } else if constexpr (LBL || MT > 2 || KI > 32) {
+                // (three or more tiles: a step's 9+ products keep the matrix cores busy)
 #pragma clang loop unroll(disable)
                 for (int j = j_a; j <= j_b; ++j) step(j);
+            } else {
+                // one or two tiles: pairs of element steps, unrolled -- phase words in fixed
+                // registers and two independent chains for the vector ALUs.  A pair that
+                // meets [j_a, j_b] runs whole; its other step lies outside the support of
+                // every voxel of the wave, where the support test (or, for WC_COS_RF, the
+                // window's cut) gives zero weight.
+                static_assert(SJC % 2 == 0, "element steps per chunk come in pairs");
+#pragma unroll
+                for (int pp = 0; pp < SJC / 2; ++pp) {
+                    const int j0 = cc * SJC + 2 * pp;
+                    if (j0 + 1 >= j_a && j0 <= j_b) {
+                        step(j0);
+                        step(j0 + 1);
+                    }
+                }
             }
             if (pipe) chunk_end(cc_const, b);
             };  // chunk_step
@@ -1117,11 +1152,12 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * (mt0 + mt) + 4 * g + r;
-                    const int f = row < A.n_tx ? row / A.rows_per_frame : -1;
+                    const int f = row < A.n_tx ? (A.row_frame ? A.row_frame[row] : row / A.rows_per_frame) : -1;
                     if (f != f_run) {
                         if (f_run >= 0) {
-                            atomicAdd(&A.image[(size_t)f_run * n_vox + v].x, A.out_scale * sr);
-                            atomicAdd(&A.image[(size_t)f_run * n_vox + v].y, A.out_scale * si);
+                            float2 *img = A.frame_image ? A.frame_image[f_run] : A.image + (size_t)f_run * n_vox;
+                            atomicAdd(&img[v].x, A.out_scale * sr);
+                            atomicAdd(&img[v].y, A.out_scale * si);
                         }
                         f_run = f;
                         sr = si = 0.f;
@@ -1130,8 +1166,9 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                     si += Ei[ENS ? mt : 0][r];
                 }
                 if (f_run >= 0) {
-                    atomicAdd(&A.image[(size_t)f_run * n_vox + v].x, A.out_scale * sr);
-                    atomicAdd(&A.image[(size_t)f_run * n_vox + v].y, A.out_scale * si);
+                    float2 *img = A.frame_image ? A.frame_image[f_run] : A.image + (size_t)f_run * n_vox;
+                    atomicAdd(&img[v].x, A.out_scale * sr);
+                    atomicAdd(&img[v].y, A.out_scale * si);
                 }
             }
             if (g == 0 && b_end == A.n_f && bz == 0 && A.weights) A.weights[v] = w_last;

@@ -105,6 +105,7 @@ PROTOTYPES = {
     "fn_abi_version": (C.c_int32, []),
     "fn_status_id": (C.c_char_p, [C.c_int32]),
     "fn_last_error": (C.c_char_p, []),
+    "fn_last_report": (C.c_char_p, []),
     "fn_device_count": (C.c_int32, [C.POINTER(C.c_int32)]),
     "fn_pw_frequency_grid": (C.c_int32, [C.POINTER(PwParams)] + [C.POINTER(C.c_int32)] * 3),
     "fn_sa_frequency_grid": (C.c_int32, [C.POINTER(SaParams)] + [C.POINTER(C.c_int32)] * 3),
@@ -132,6 +133,8 @@ PROTOTYPES = {
                                      C.POINTER(C.c_void_p), C.c_void_p]),
     "fn_batch_info": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "fn_batch_set_persistent": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "fn_batch_set_stacking": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "fn_batch_stacks": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "fn_batch_destroy": (C.c_int32, [C.c_void_p]),
     "fn_plan_focused_signal": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fn_plan_set_tof": (C.c_int32, [C.c_void_p, C.c_double, _dp]),

@@ -9,11 +9,13 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdarg>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 namespace fnk {
@@ -141,6 +143,7 @@ struct fn_plan {
     int executed = 0;           // spectra of an execute are in d_X (focused signal)
     float2 *d_sigF = nullptr;   // focused spectrum of the signal voxel [n_f]
     std::vector<double> h_px, h_pz, h_Frx, h_Ftx, h_sqf;  // host copies (signal voxel search)
+    std::vector<double> h_delay;                          // delay offsets of the last set_tof (empty: zeros)
     int precision = FN_PRECISION_FP32;
     int verbosity = 0;
     fnk::BeamArgs args{};
@@ -473,6 +476,7 @@ int configure_common(fn_plan *P, const Common &c, const fnh::Grid &g)
     fnk::SpecArgs &S = P->sargs;
     S.n_t = c.n_t; S.n_ch = c.n_el * c.n_tx; S.N = g.N; S.n_lb = g.n_lb; S.n_f = g.n_f;
     S.n_el = c.n_el; S.n_tx = c.n_tx; S.MT = P->MT_all; S.KI = P->KI;
+    S.row0 = 0; S.MT_out = P->MT_all; S.write_pad = 1;
 
     P->flops = 8.0 * g.n_f * (double)P->n_vox * c.n_el * (P->lbl ? 1 : c.n_tx);
     return FN_OK;
@@ -566,10 +570,10 @@ hipError_t lds_limit(const void *kern, size_t lds, size_t &done)
     return e;
 }
 
-int launch_spectra(fn_plan *P, const void *d_rf, int dtype, hipStream_t s)
+// the band spectra of RF d_rf into the operand layout X described by S
+int launch_spectra_into(const fn_plan *P, const void *d_rf, int dtype, hipStream_t s, float2 *X,
+                        const fnk::SpecArgs &S)
 {
-    P->executed = 1;
-    const fnk::SpecArgs &S = P->sargs;
     // 64 bins x one operand group (16 rows x 4 elements) per block
     dim3 sg((unsigned)((S.n_f + 63) / 64), (unsigned)(S.MT * S.KI));
     static size_t done_f = 0, done_d = 0;
@@ -579,13 +583,19 @@ int launch_spectra(fn_plan *P, const void *d_rf, int dtype, hipStream_t s)
               "spectra LDS limit");
     if (dtype == FN_DTYPE_F32) {
         hipLaunchKernelGGL(fnk::spectra_kernel<float>, sg, dim3(256), P->lds_spec, s,
-                           reinterpret_cast<const float *>(d_rf), P->d_tw, P->d_X, S);
+                           reinterpret_cast<const float *>(d_rf), P->d_tw, X, S);
     } else {
         hipLaunchKernelGGL(fnk::spectra_kernel<double>, sg, dim3(256), P->lds_spec, s,
-                           reinterpret_cast<const double *>(d_rf), P->d_tw, P->d_X, S);
+                           reinterpret_cast<const double *>(d_rf), P->d_tw, X, S);
     }
     HIP_CHECK(hipGetLastError(), "spectra_kernel");
     return FN_OK;
+}
+
+int launch_spectra(fn_plan *P, const void *d_rf, int dtype, hipStream_t s)
+{
+    P->executed = 1;
+    return launch_spectra_into(P, d_rf, dtype, s, P->d_X, P->sargs);
 }
 
 // the beamforming arguments of one execute (image and weight buffers)
@@ -604,6 +614,27 @@ fnk::BeamArgs beam_args(const fn_plan *P, float *d_image, float *d_w, float *d_w
         A.weights_tx = d_w;
     }
     return A;
+}
+
+// the 16-bit operand planes (and per-bin scales) of FP32 spectra X of MT tiles;
+// nothing for FP32 plans
+int launch_lowp_planes(int precision, const float2 *X, void *Xlp, float *xscale, int n_f, int MT, int KI, hipStream_t s)
+{
+    if (precision == FN_PRECISION_FP32) return FN_OK;
+    const dim3 cg((unsigned)n_f), cb(256);
+    switch (precision) {
+    case FN_PRECISION_BF16X3:
+        hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_BF16X3>, cg, cb, 0, s, X, Xlp, xscale, MT, KI);
+        break;
+    case FN_PRECISION_FP16X3:
+        hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_FP16X3>, cg, cb, 0, s, X, Xlp, xscale, MT, KI);
+        break;
+    default:
+        hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_FP16>, cg, cb, 0, s, X, Xlp, xscale, MT, KI);
+        break;
+    }
+    HIP_CHECK(hipGetLastError(), "lowp_planes_kernel");
+    return FN_OK;
 }
 
 // after the beamforming launch: the partial images of a voxel's workgroups summed
@@ -629,22 +660,7 @@ int prepare(fn_plan *P, const void *d_rf, int dtype, float *d_image, hipStream_t
     }
     const fnk::SpecArgs &S = P->sargs;
     if (int st = launch_spectra(P, d_rf, dtype, s)) return st;
-    if (P->precision != FN_PRECISION_FP32) {
-        const dim3 cg((unsigned)S.n_f), cb(256);
-        switch (P->precision) {
-        case FN_PRECISION_BF16X3:
-            hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_BF16X3>, cg, cb, 0, s, P->d_X, P->d_Xlp, P->d_xscale, S.MT, S.KI);
-            break;
-        case FN_PRECISION_FP16X3:
-            hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_FP16X3>, cg, cb, 0, s, P->d_X, P->d_Xlp, P->d_xscale, S.MT, S.KI);
-            break;
-        default:
-            hipLaunchKernelGGL(fnk::lowp_planes_kernel<fnk::PREC_FP16>, cg, cb, 0, s, P->d_X, P->d_Xlp, P->d_xscale, S.MT, S.KI);
-            break;
-        }
-        HIP_CHECK(hipGetLastError(), "lowp_planes_kernel");
-    }
-    return FN_OK;
+    return launch_lowp_planes(P->precision, P->d_X, P->d_Xlp, P->d_xscale, S.n_f, S.MT, S.KI, s);
 }
 
 int execute(fn_plan *P, const void *d_rf, int dtype, float *d_image, float *d_w, float *d_w_rx, hipStream_t user)
@@ -902,6 +918,7 @@ int set_tof(fn_plan *P, double c_tof, const double *delays)
     // kernels of earlier calls on any stream may still read the delay offsets
     HIP_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
     HIP_CHECK(hipMemcpy(P->d_delay, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice), "delay offsets");
+    P->h_delay = d;
     P->args.ctof = c_tof > 0 ? c_tof : P->args.c0;
     ++P->tof_serial;
     return FN_OK;
@@ -1116,29 +1133,143 @@ int focused_signal(fn_plan *P, float *d_signal, hipStream_t s)
     return FN_OK;
 }
 
-// one-shot: H2D, execute, D2H (gpu_bf_das_pw_rf.cu:730-1122)
+// run-time report of the last one-shot call of this thread with verbosity >= 1:
+// the device, the launch parameters, the memory in use and per-stage times (the
+// sections gpu_bf_das_pw_rf.cu:664-689, 875-903, 1128-1144 print); the gateways
+// print it after their own status section
+thread_local std::string t_report;
+
+void report_line(std::string &r, const char *fmt, ...)
+{
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    r += buf;
+}
+
+const char *kRule = " --------------------------------------------------------------------------------\n";
+
+void report_bytes(std::string &r, const char *what, double bytes, double total)
+{
+    report_line(r, "     %-31s: %12.2f KiByte (%6.2f %%)\n", what, bytes / 1024.0, total > 0 ? 100.0 * bytes / total : 0.0);
+}
+
+void report_time(std::string &r, const char *what, double ms, double total)
+{
+    report_line(r, "     %-34s: %10.3f ms (%6.2f %%)\n", what, ms, total > 0 ? 100.0 * ms / total : 0.0);
+}
+
+void compose_report(const fn_plan *P, size_t rf_bytes, size_t img_bytes, size_t w_bytes, int n_w, const float *ms)
+{
+    std::string &r = t_report;
+    r.clear();
+    hipDeviceProp_t prop{};
+    int count = 0, drv = 0, rt = 0;
+    size_t mem_free = 0, mem_total = 0;
+    hipGetDeviceProperties(&prop, P->device);
+    hipGetDeviceCount(&count);
+    hipDriverGetVersion(&drv);
+    hipRuntimeGetVersion(&rt);
+    hipMemGetInfo(&mem_free, &mem_total);
+    static const char *kPrec[] = {"fp32", "bf16x3", "fp16x3", "fp16"};
+    static const char *kWin[] = {"range", "linear", "cosine", "cosine, full array", "cosine, cut at the aperture"};
+    r += kRule;
+    report_line(r, " Selected device %d of %d: %s (%s)\n", P->device, count, prop.name, prop.gcnArchName);
+    r += kRule;
+    report_line(r, "  %-20s: %-18d %-22s: %d\n", "compute units", prop.multiProcessorCount, "wavefront size", prop.warpSize);
+    report_line(r, "  %-20s: %-11.2f MiByte %-22s: %.2f KiByte\n", "total global memory",
+                (double)prop.totalGlobalMem / 1048576.0, "LDS per compute unit",
+                (double)prop.maxSharedMemoryPerMultiProcessor / 1024.0);
+    report_line(r, "  %-20s: %-18d %-22s: %d.%d / %d.%d\n", "registers per block", prop.regsPerBlock,
+                "driver / runtime", drv / 10000000, (drv / 100000) % 100, rt / 10000000, (rt / 100000) % 100);
+    report_line(r, "  %-20s: %-18s %-22s: %s\n", "contraction", kPrec[P->precision], "mode",
+                P->mode == 0 ? (P->n_frames > 1 ? "plane waves, ensemble" : "plane waves") : "synthetic aperture");
+    r += kRule;
+    r += " launch parameters\n";
+    r += kRule;
+    const long tiles = beam_tiles(P);
+    report_line(r, "  %-20s: %-6ld x 64 voxels %-11s %-22s: %d x %d of 16 rows\n", "voxel tiles", tiles, "",
+                "transmit groups x tiles", P->n_groups, P->MT_tmpl);
+    report_line(r, "  %-20s: %-3d of %-5d bins %-8s %-22s: %d (4 elements each)\n", "bin chunks", P->bin_split,
+                P->args.bin_chunk, "", "element steps / lane", P->KI);
+    report_line(r, "  %-20s: %-18ld %-22s: %s\n", "workgroups", tiles * P->bin_split * P->n_groups, "receive window class",
+                kWin[P->wcr]);
+    const double b_pos = 8.0 * (P->n_x + P->n_z);
+    const double b_tab = 8.0 * P->g.n_f * (P->d_Ftx ? 3 : 2) + (P->d_txp ? 24.0 * P->n_tx : 0.0) + 8.0 * P->g.N +
+                         8.0 * P->n_el;
+    const double b_spec = (double)P->x_bytes + (double)P->xlp_bytes + (P->xlp_bytes ? 4.0 * P->g.n_f : 0.0);
+    const double b_part = 8.0 * (double)P->pimg_cap;
+    const double b_tot = b_pos + b_tab + rf_bytes + b_spec + b_part + img_bytes + (double)w_bytes * n_w;
+    r += kRule;
+    report_line(r, " memory consumption (available global memory: %.2f MiByte (%.2f %%))\n", mem_free / 1048576.0,
+                mem_total ? 100.0 * mem_free / mem_total : 0.0);
+    r += kRule;
+    report_bytes(r, "voxel positions", b_pos, b_tot);
+    report_bytes(r, "F-numbers, steering, twiddles", b_tab, b_tot);
+    report_bytes(r, "RF signals", (double)rf_bytes, b_tot);
+    report_bytes(r, "band spectra (workspace)", b_spec, b_tot);
+    if (b_part > 0) report_bytes(r, "partial images", b_part, b_tot);
+    report_bytes(r, "DAS image", (double)img_bytes, b_tot);
+    if (n_w) report_bytes(r, "weights", (double)w_bytes * n_w, b_tot);
+    report_line(r, "     %-31s  %s\n", "-------------------------------", "----------------------------");
+    report_bytes(r, "total", b_tot, b_tot);
+    const double t_tot = ms[0] + ms[1] + ms[2] + ms[3];
+    r += kRule;
+    r += " performance statistics:\n";
+    r += kRule;
+    report_time(r, "memory transfer: host to device", ms[0], t_tot);
+    report_time(r, "band-limited DFT (spectra)", ms[1], t_tot);
+    report_time(r, "beamforming kernel", ms[2], t_tot);
+    report_time(r, "memory transfer: device to host", ms[3], t_tot);
+    report_line(r, "     %-34s  %s\n", "----------------------------------", "------------------------");
+    report_time(r, "total", t_tot, t_tot);
+}
+
+// one-shot: H2D, execute, D2H (gpu_bf_das_pw_rf.cu:730-1122); with verbosity >= 1
+// the stages are timed with HIP events and reported
 int one_shot(fn_plan *P, const void *rf, int dtype, float *image, float *w0, float *w1)
 {
     const size_t rf_bytes = (size_t)P->n_t * P->n_el * P->n_tx * dtype_size(dtype);
     const size_t img_bytes = (size_t)P->n_vox * 2 * sizeof(float) * P->n_frames;
     const size_t w_bytes = (size_t)P->n_vox * sizeof(float);
+    const bool rep = P->verbosity > 0;
     void *d_rf = nullptr;
     float *d_img = nullptr, *d_w0 = nullptr, *d_w1 = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int s = FN_OK;
     hipError_t e = hipMalloc(&d_rf, rf_bytes);
     if (e == hipSuccess) e = hipMalloc(&d_img, img_bytes);
     if (e == hipSuccess && w0) e = hipMalloc(&d_w0, w_bytes);
     if (e == hipSuccess && w1) e = hipMalloc(&d_w1, w_bytes);
+    for (int i = 0; i < 4 && rep && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+    if (rep) P->profiling = 1;
+    if (e == hipSuccess && rep) e = hipEventRecord(ev[0], P->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_rf, rf, rf_bytes, hipMemcpyHostToDevice, P->stream);
+    if (e == hipSuccess && rep) e = hipEventRecord(ev[1], P->stream);
     if (e != hipSuccess) s = hip_fail(e, "one-shot buffers");
     if (!s) s = execute(P, d_rf, dtype, d_img, d_w0, d_w1, P->stream);
     if (!s) {
-        e = hipMemcpyAsync(image, d_img, img_bytes, hipMemcpyDeviceToHost, P->stream);
+        e = rep ? hipEventRecord(ev[2], P->stream) : hipSuccess;
+        if (e == hipSuccess) e = hipMemcpyAsync(image, d_img, img_bytes, hipMemcpyDeviceToHost, P->stream);
         if (e == hipSuccess && w0) e = hipMemcpyAsync(w0, d_w0, w_bytes, hipMemcpyDeviceToHost, P->stream);
         if (e == hipSuccess && w1) e = hipMemcpyAsync(w1, d_w1, w_bytes, hipMemcpyDeviceToHost, P->stream);
+        if (e == hipSuccess && rep) e = hipEventRecord(ev[3], P->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
         if (e != hipSuccess) s = hip_fail(e, "one-shot results");
     }
+    if (!s && rep) {
+        float ms[4] = {0.f, 0.f, 0.f, 0.f};
+        hipEventElapsedTime(&ms[0], ev[0], ev[1]);
+        hipEventElapsedTime(&ms[3], ev[2], ev[3]);
+        collect_profile(P);
+        ms[1] = (float)P->spec_ms;
+        ms[2] = (float)P->beam_ms;
+        compose_report(P, rf_bytes, img_bytes, w_bytes, (w0 ? 1 : 0) + (w1 ? 1 : 0), ms);
+    }
+    for (auto &x : ev)
+        if (x) hipEventDestroy(x);
     hipFree(d_rf); hipFree(d_img); hipFree(d_w0); hipFree(d_w1);
     return s;
 }
@@ -1466,9 +1597,12 @@ int32_t fn_plan_destroy(fn_plan *P)
     return FN_OK;
 }
 
+const char *fn_last_report(void) { return t_report.c_str(); }
+
 int32_t fn_das_pw(const fn_pw_params *p, const void *data_rf, int32_t rf_dtype, float *image, float *weights)
 {
     fnh::clear_error();
+    t_report.clear();
     if (!data_rf) return fail(FN_ERR_RF_DATA, "RF data missing");
     if (!image) return fail(FN_ERR_ARGUMENT, "image buffer missing");
     if (rf_dtype != FN_DTYPE_F32 && rf_dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
@@ -1514,6 +1648,7 @@ int32_t fn_das_sa(const fn_sa_params *p, const void *data_rf, int32_t rf_dtype, 
                   float *weights_rx)
 {
     fnh::clear_error();
+    t_report.clear();
     if (!data_rf) return fail(FN_ERR_RF_DATA, "RF data missing");
     if (!image) return fail(FN_ERR_ARGUMENT, "image buffer missing");
     if (rf_dtype != FN_DTYPE_F32 && rf_dtype != FN_DTYPE_F64) return fail(FN_ERR_RF_DATA, "RF data must be float32 or float64");
@@ -1537,10 +1672,33 @@ struct BatchClass {
     int n_items = 0, grid = 0;
 };
 
+// Row stack: plane-wave plans of a batch that differ only in their transmit rows
+// (steering angles) and RF share every receive-side quantity -- voxel grid,
+// frequency grid, F-numbers, window, normalization, element geometry and delays.
+// Their rows are stacked into one operand (16 rows per matrix-core tile, a plan's
+// rows need not start a tile) and beamformed by one ensemble launch whose epilogue
+// adds every row to its own plan's image.  The apodization x phase values, one per
+// (voxel, bin, element) generated on the vector ALUs, then serve all stacked rows
+// of a transmit group (up to 64) instead of the few rows of one plan.
+struct RowStack {
+    std::vector<int> plans;  // members (batch indices), stacked in this order
+    std::vector<int> row0;   // first stacked row of each member
+    int rows = 0, MT_all = 0, MT_tmpl = 0, n_groups = 0, bin_split = 1, chunk = 0;
+    size_t lds_beam = 0;
+    float2 *d_X = nullptr;
+    void *d_Xlp = nullptr;
+    float *d_xscale = nullptr;
+    double *d_txp = nullptr;
+    int *d_row_frame = nullptr;
+    float2 **d_frame_image = nullptr;
+};
+
 struct fn_batch {
     std::vector<fn_plan *> plans;
     int device = 0;
     int persistent = 0;                // plans sharing a kernel class run as one persistent launch
+    int stacking = 1;                  // row stacks of plans that share their receive side
+    std::vector<RowStack> stacks;
     std::vector<BatchClass> classes;
     std::vector<int> singles;          // plans launched on their own
     hipStream_t pstream = nullptr;     // stream of the persistent launches inside the capture
@@ -1568,8 +1726,148 @@ static void batch_release_graph(fn_batch *b)
         hipFree(c.d_items);
         hipFree(c.d_counter);
     }
+    for (auto &r : b->stacks) {
+        hipFree(r.d_X);
+        hipFree(r.d_Xlp);
+        hipFree(r.d_xscale);
+        hipFree(r.d_txp);
+        hipFree(r.d_row_frame);
+        hipFree(r.d_frame_image);
+    }
+    b->stacks.clear();
     b->classes.clear();
     b->singles.clear();
+}
+
+// plans whose rows can share one beamforming launch: everything but the transmit
+// rows and the RF is equal (single-frame plane-wave plans up to 128 elements)
+static bool row_stackable(const fn_plan *P)
+{
+    // beyond three tiles of rows a plan fills its own transmit groups (the ensemble
+    // kernel holds at most four tiles per group against five of the plain kernel)
+    return P->mode == 0 && P->n_frames == 1 && !P->lbl && !P->nw && P->KI <= 32 && P->n_tx <= 48;
+}
+
+static bool same_receive_side(const fn_plan *a, const fn_plan *b)
+{
+    const fnk::BeamArgs &x = a->args, &y = b->args;
+    auto zeros = [](const std::vector<double> &v) { return std::all_of(v.begin(), v.end(), [](double d) { return d == 0.0; }); };
+    const bool delays = (a->h_delay.empty() || zeros(a->h_delay)) && (b->h_delay.empty() || zeros(b->h_delay))
+                            ? true
+                            : a->h_delay == b->h_delay;
+    return a->device == b->device && a->precision == b->precision && a->n_x == b->n_x && a->n_z == b->n_z &&
+           a->n_t == b->n_t && a->n_el == b->n_el && a->KI == b->KI && a->g.N == b->g.N && a->g.n_lb == b->g.n_lb &&
+           a->g.n_f == b->g.n_f && a->wcr == b->wcr && a->h_px == b->h_px && a->h_pz == b->h_pz &&
+           a->h_Frx == b->h_Frx && delays && std::memcmp(&x.wrx, &y.wrx, sizeof(x.wrx)) == 0 &&
+           x.support_rx == y.support_rx && x.norm_rx == y.norm_rx && x.clamp == y.clamp &&
+           x.out_scale == y.out_scale && x.tx_kind == y.tx_kind && x.pitch == y.pitch && x.width == y.width &&
+           x.c0 == y.c0 && x.ctof == y.ctof && x.f_s == y.f_s && x.t0 == y.t0 && x.M == y.M;
+}
+
+// group the stackable plans (two or more per stack), size each stack's tiling and
+// build its tables: stacked steering rows, row -> plan, plan -> image; outside any capture
+static int batch_plan_stacks(fn_batch *b, float *const *d_images, std::vector<char> &taken)
+{
+    const int n = (int)b->plans.size();
+    for (int i = 0; i < n; ++i) {
+        if (taken[i] || !row_stackable(b->plans[i])) continue;
+        RowStack r;
+        for (int j = i; j < n; ++j)
+            if (!taken[j] && row_stackable(b->plans[j]) && same_receive_side(b->plans[i], b->plans[j]))
+                r.plans.push_back(j);
+        if (r.plans.size() < 2) continue;
+        for (int j : r.plans) {
+            r.row0.push_back(r.rows);
+            r.rows += b->plans[j]->n_tx;
+            taken[j] = 1;
+        }
+        const fn_plan *P0 = b->plans[i];
+        r.MT_all = (r.rows + 15) / 16;
+        const int max_tiles = tile_cap(2, P0->precision);  // the ensemble kernel's register budget
+        const int groups = (r.MT_all + max_tiles - 1) / max_tiles;
+        r.MT_tmpl = (r.MT_all + groups - 1) / groups;
+        r.n_groups = (r.MT_all + r.MT_tmpl - 1) / r.MT_tmpl;
+        const int planes = P0->precision == FN_PRECISION_FP16 ? 2 : 4;
+        const size_t step_bytes = P0->precision == FN_PRECISION_FP32 ? 64 * sizeof(float2) : (size_t)planes * 64 * 2;
+        r.lds_beam = (size_t)r.MT_tmpl * P0->KI * step_bytes;
+        // bin chunks: enough workgroups to fill the GPU several times over
+        const long blocks = beam_tiles(P0) * r.n_groups;
+        int split = 1;
+        if (blocks < 2048) split = (int)std::min<long>((2048 + blocks - 1) / blocks, std::max(1, P0->g.n_f / 8));
+        r.chunk = (P0->g.n_f + split - 1) / split;
+        r.bin_split = (P0->g.n_f + r.chunk - 1) / r.chunk;
+        b->stacks.push_back(r);  // freed with the graph, also on failure
+        RowStack &R = b->stacks.back();
+        const size_t n_f = (size_t)P0->g.n_f;
+        const size_t x_bytes = n_f * R.MT_all * P0->KI * 64 * sizeof(float2);
+        std::vector<int> rf(16 * (size_t)R.MT_all, -1);
+        std::vector<float2 *> imgs;
+        for (size_t m = 0; m < R.plans.size(); ++m) {
+            for (int a = 0; a < b->plans[R.plans[m]]->n_tx; ++a) rf[R.row0[m] + a] = (int)m;
+            imgs.push_back(reinterpret_cast<float2 *>(d_images[R.plans[m]]));
+        }
+        hipError_t e = hipMalloc(&R.d_X, x_bytes);
+        if (e == hipSuccess) e = hipMemset(R.d_X, 0, x_bytes);  // rows past the last plan stay zero
+        if (e == hipSuccess && P0->precision != FN_PRECISION_FP32) {
+            e = hipMalloc(&R.d_Xlp, n_f * planes * R.MT_all * P0->KI * 64 * 2);
+            if (e == hipSuccess) e = hipMalloc(&R.d_xscale, n_f * sizeof(float));
+        }
+        if (e == hipSuccess) e = hipMalloc(&R.d_txp, 3 * sizeof(double) * R.rows);
+        for (size_t m = 0; m < R.plans.size() && e == hipSuccess; ++m) {
+            const fn_plan *P = b->plans[R.plans[m]];
+            e = hipMemcpy(R.d_txp + 3 * (size_t)R.row0[m], P->d_txp, 3 * sizeof(double) * P->n_tx, hipMemcpyDeviceToDevice);
+        }
+        if (e == hipSuccess) e = hipMalloc(&R.d_row_frame, rf.size() * sizeof(int));
+        if (e == hipSuccess) e = hipMemcpy(R.d_row_frame, rf.data(), rf.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&R.d_frame_image, imgs.size() * sizeof(float2 *));
+        if (e == hipSuccess)
+            e = hipMemcpy(R.d_frame_image, imgs.data(), imgs.size() * sizeof(float2 *), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "row stack tables");
+    }
+    return FN_OK;
+}
+
+// one stack on stream s: the members' images zeroed, their spectra into the stacked
+// operand (each at its first row), the 16-bit planes, one ensemble launch
+static int batch_run_stack(fn_batch *b, const RowStack &R, const void *const *d_rf, const int32_t *dtypes,
+                           float *const *d_images, hipStream_t s)
+{
+    const fn_plan *P0 = b->plans[R.plans[0]];
+    for (size_t m = 0; m < R.plans.size(); ++m) {
+        fn_plan *P = b->plans[R.plans[m]];
+        const long nz = 2 * P->n_vox;
+        hipLaunchKernelGGL(fnk::fill_zero_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s,
+                           d_images[R.plans[m]], nz);
+        HIP_CHECK(hipGetLastError(), "fill_zero_kernel");
+        fnk::SpecArgs S = P->sargs;
+        S.row0 = R.row0[m];
+        S.MT_out = R.MT_all;
+        S.write_pad = 0;
+        if (int st = launch_spectra_into(P, d_rf[R.plans[m]], dtypes[R.plans[m]], s, R.d_X, S)) return st;
+    }
+    if (int st = launch_lowp_planes(P0->precision, R.d_X, R.d_Xlp, R.d_xscale, P0->g.n_f, R.MT_all, P0->KI, s))
+        return st;
+    fnk::BeamArgs A = P0->args;
+    A.X = R.d_X;
+    A.Xlp = R.d_Xlp;
+    A.xscale = R.d_xscale;
+    A.txp = R.d_txp;
+    A.n_tx = R.rows;
+    A.tx_group = R.MT_tmpl;
+    A.bin_chunk = R.chunk;
+    A.rows_per_frame = 0;
+    A.row_frame = R.d_row_frame;
+    A.frame_image = R.d_frame_image;
+    A.image = nullptr;
+    A.weights = nullptr;
+    A.weights_tx = nullptr;
+    A.partial = nullptr;
+    A.n_slots = 1;
+    A.slot_z = R.n_groups;
+    const dim3 grid((unsigned)beam_tiles(P0), (unsigned)R.bin_split, (unsigned)R.n_groups);
+    hipError_t e = fnk::kEnsLaunchers[ki_index(P0->KI)][P0->precision](R.MT_tmpl, P0->wcr, grid, R.lds_beam, s, A);
+    if (e != hipSuccess) return hip_fail(e, "beamform_kernel (row stack)");
+    return FN_OK;
 }
 
 // kernel class of a plan for the persistent launches; -1: launched on its own
@@ -1584,8 +1882,12 @@ static long batch_class_key(const fn_plan *P)
 // size its grid (resident workgroups x CUs); outside any capture
 static int batch_plan_classes(fn_batch *b, float *const *d_images)
 {
+    std::vector<char> taken(b->plans.size(), 0);
+    if (b->stacking)
+        if (int st = batch_plan_stacks(b, d_images, taken)) return st;
     std::vector<std::pair<long, std::vector<int>>> groups;
     for (int i = 0; i < (int)b->plans.size(); ++i) {
+        if (taken[i]) continue;
         const long key = b->persistent ? batch_class_key(b->plans[i]) : -1;
         if (key < 0) { b->singles.push_back(i); continue; }
         auto it = std::find_if(groups.begin(), groups.end(), [&](const auto &g) { return g.first == key; });
@@ -1704,15 +2006,23 @@ int32_t fn_batch_execute(fn_batch *b, const void *const *d_rf, const int32_t *dt
         }
         HIP_CHECK(hipStreamSynchronize(s), "batch warm-up");
         if (int st = batch_plan_classes(b, d_images)) { batch_release_graph(b); return st; }
+        // the row stacks once outside the capture (raises their kernels' LDS limits)
+        for (const RowStack &R : b->stacks)
+            if (int st = batch_run_stack(b, R, d_rf, dtypes, d_images, s)) { batch_release_graph(b); return st; }
+        if (!b->stacks.empty()) HIP_CHECK(hipStreamSynchronize(s), "row stack warm-up");
         HIP_CHECK(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         int st = FN_OK;
         const size_t W = b->streams.size();
         hipError_t e = hipEventRecord(b->fork, b->capture);
         for (size_t w = 0; w < W && e == hipSuccess; ++w) e = hipStreamWaitEvent(b->streams[w], b->fork, 0);
-        // plans on their own (ensembles and the like, usually the longest) first, one
-        // worker each; the persistent classes' spectra on the other workers
-        const size_t n_single = b->singles.size();
+        // row stacks and plans on their own (ensembles and the like, usually the longest)
+        // first, one worker each; the persistent classes' spectra on the other workers
+        const size_t n_single = b->stacks.size() + b->singles.size();
         size_t slot = 0;
+        for (const RowStack &R : b->stacks) {
+            if (e != hipSuccess || st) break;
+            st = batch_run_stack(b, R, d_rf, dtypes, d_images, b->streams[slot++ % W]);
+        }
         for (int i : b->singles) {
             if (e != hipSuccess || st) break;
             fn_plan *P = b->plans[i];
@@ -1800,6 +2110,28 @@ int32_t fn_batch_destroy(fn_batch *b)
     if (b->capture) hipStreamDestroy(b->capture);
     if (b->pstream) hipStreamDestroy(b->pstream);
     delete b;
+    return FN_OK;
+}
+
+int32_t fn_batch_set_stacking(fn_batch *b, int32_t enable)
+{
+    fnh::clear_error();
+    if (!b) return fail(FN_ERR_ARGUMENT, "null batch");
+    if (b->stacking != (enable != 0)) {
+        hipSetDevice(b->device);
+        batch_release_graph(b);  // captured again at the next execute
+    }
+    b->stacking = enable != 0;
+    return FN_OK;
+}
+
+int32_t fn_batch_stacks(const fn_batch *b, int32_t *n_stacks, int32_t *n_stacked_plans)
+{
+    if (!b) return fail(FN_ERR_ARGUMENT, "null batch");
+    int32_t k = 0;
+    for (const RowStack &r : b->stacks) k += (int32_t)r.plans.size();
+    if (n_stacks) *n_stacks = (int32_t)b->stacks.size();
+    if (n_stacked_plans) *n_stacked_plans = k;
     return FN_OK;
 }
 

@@ -111,6 +111,47 @@ inline int32_t semantics_arg(int nrhs, const mxArray *prhs[], int i, const char 
     return FN_SEMANTICS_CPU;
 }
 
+// the name MATLAB gives an object: string(object), then char() of that string,
+// as the reference's gateways name windows, F-numbers and normalizations in
+// their status section (gpu_bf_helper_functions.cu:113-138)
+inline std::string name_of(const mxArray *obj)
+{
+    mxArray *rhs = mxDuplicateArray(obj), *str = nullptr, *chr = nullptr;
+    mexCallMATLAB(1, &str, 1, &rhs, "string");
+    mexCallMATLAB(1, &chr, 1, &str, "char");
+    char buf[160] = {0};
+    mxGetString(chr, buf, sizeof(buf));
+    mxDestroyArray(rhs);
+    mxDestroyArray(str);
+    mxDestroyArray(chr);
+    return buf;
+}
+
+inline void rule(char c)
+{
+    char line[82];
+    std::memset(line, c, 80);
+    line[80] = 0;
+    mexPrintf(" %s\n", line);
+}
+
+// the status lines shared by both gateways (gpu_bf_das_pw_rf.cu:609-632): geometry,
+// sampling, medium and band; each gateway adds its transmit lines and names
+inline void status_common(const char *title, int n_x, int n_z, int n_t, int n_el, double f_s, double c_0,
+                          double f_lb, double f_ub, int32_t lb, int32_t ub, double index_t0)
+{
+    rule('=');
+    mexPrintf(" %s (libfnumber_hip, ABI %d), status information\n", title, fn_abi_version());
+    rule('=');
+    mexPrintf("  %-20s: %5d x %-5d %5s %-22s: %d\n", "lattice geometry", n_x, n_z, "", "number of samples", n_t);
+    mexPrintf("  %-20s: %-11d %7s %-22s: %5.1f MHz\n", "number rx channels", n_el, "", "sampling rate", f_s / 1e6);
+    mexPrintf("  %-20s: %7.2f m/s %7s %-22s: %5.1f MHz (%d)\n", "average sound speed", c_0, "",
+              "lower frequency bound", f_lb / 1e6, lb);
+    mexPrintf("  %-20s: %-6.2f %12s %-22s: %5.1f MHz (%d)\n", "additional shift", index_t0, "",
+              "upper frequency bound", f_ub / 1e6, ub);
+    mexPrintf("  %-20s: %-6d\n", "number of frequencies", ub - lb + 1);
+}
+
 // the image of the core (interleaved complex float [n_x][n_z]) as MATLAB's
 // complex single [N_z x N_x] with separate parts (gpu_bf_das_pw_rf.cu:1077-1122)
 inline mxArray *image_out(const std::vector<float> &img, int n_z, int n_x)

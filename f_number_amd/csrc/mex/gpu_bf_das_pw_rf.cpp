@@ -122,15 +122,37 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     std::vector<double> r(ub - lb + 1);
     check(fn_element_pitch_over_lambda(p.element_pitch, p.f_s, p.c_0, N, lb, ub, r.data()), GW);
     std::vector<double> Fv = compute_values(F, r);
+    if (p.verbosity > 0) {
+        // status section (gpu_bf_das_pw_rf.cu:609-632), before the computation
+        status_common("gpu_bf_das_pw_rf: DAS, steered plane waves", p.n_x, p.n_z, n_t, n_el, p.f_s, p.c_0, p.f_lb,
+                      p.f_ub, lb, ub, p.index_t0);
+        double a0 = p.steering_angles[0], a1 = a0;
+        for (int a = 1; a < n_a; ++a) {
+            a0 = std::fmin(a0, p.steering_angles[a]);
+            a1 = std::fmax(a1, p.steering_angles[a]);
+        }
+        const double deg = 180.0 / 3.14159265358979323846;
+        mexPrintf("  %-20s: %d, %.2f..%.2f deg %-4s %-22s: %d\n", "angles of incidence", n_a, a0 * deg, a1 * deg, "",
+                  "DFT order", N);
+        // the transmit reference is the array edge the wave reaches first (gpu_bf_das_pw_rf.cu:519)
+        const double x_ref = 0.5 * (n_el - 1) * p.element_pitch * 1e3;
+        if (n_a == 1)
+            mexPrintf("  %-20s: %7.2f mm\n", "reference position", -std::sin(a0) >= 0 ? -x_ref : x_ref);
+        else
+            mexPrintf("  %-20s: +-%.2f mm (by the sign of each angle)\n", "reference position", x_ref);
+        mexPrintf("  %-20s: %s\n", "window", present(nrhs, prhs, 10) ? name_of(prhs[10]).c_str() : "boxcar");
+        mexPrintf("  %-20s: %s\n", "F-number", name_of(F).c_str());
+        mexPrintf("  %-20s: %s\n", "normalization", present(nrhs, prhs, 12) ? name_of(prhs[12]).c_str() : "on");
+        mexPrintf("  %-20s: %s\n", "semantics", p.semantics == FN_SEMANTICS_MEX ? "mex" : "cpu");
+    }
     if (own_F) mxDestroyArray(F);
     p.f_number_values = Fv.data();
     p.n_f = (int32_t)Fv.size();
 
     std::vector<float> img(2 * (size_t)p.n_x * p.n_z), w((size_t)p.n_x * p.n_z);
     check(fn_das_pw(&p, mxGetPr(prhs[2]), FN_DTYPE_F64, img.data(), w.data()), GW);
-    if (p.verbosity > 0)
-        mexPrintf("gpu_bf_das_pw_rf: %d x %d voxels, %d elements, %d angle(s), N_dft = %d, bins %d..%d\n",
-                  p.n_z, p.n_x, n_el, n_a, N, lb, ub);
+    // device, launch, memory and timing sections (gpu_bf_das_pw_rf.cu:664-689, 875-903, 1128-1144)
+    if (p.verbosity > 0) mexPrintf("%s", fn_last_report());
     plhs[0] = image_out(img, p.n_z, p.n_x);
     if (nlhs > 1) plhs[1] = weights_out(w, p.n_z, p.n_x);
 }

@@ -129,6 +129,19 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     std::vector<double> r(ub - lb + 1);
     check(fn_element_pitch_over_lambda(p.element_pitch, p.f_s, p.c_0, N, lb, ub, r.data()), GW);
     std::vector<double> Ft = compute_values(Ftx, r), Fr = compute_values(Frx, r);
+    if (p.verbosity > 0) {
+        // status section (gpu_bf_saft_rf.cu, as gpu_bf_das_pw_rf.cu:609-632), before the computation
+        status_common("gpu_bf_saft_rf: DAS, synthetic aperture", p.n_x, p.n_z, p.n_samples, p.n_elements, p.f_s, p.c_0,
+                      p.f_lb, p.f_ub, lb, ub, p.index_t0);
+        mexPrintf("  %-20s: %-6d %12s %-22s: %d\n", "number tx channels", p.n_elements, "", "DFT order", N);
+        mexPrintf("  %-20s: %s\n", "window (tx)", present(nrhs, prhs, 9) ? name_of(prhs[9]).c_str() : "boxcar");
+        mexPrintf("  %-20s: %s\n", "window (rx)", present(nrhs, prhs, 10) ? name_of(prhs[10]).c_str() : "boxcar");
+        mexPrintf("  %-20s: %s\n", "F-number (tx)", name_of(Ftx).c_str());
+        mexPrintf("  %-20s: %s\n", "F-number (rx)", name_of(Frx).c_str());
+        mexPrintf("  %-20s: %s\n", "normalization (tx)", present(nrhs, prhs, 13) ? name_of(prhs[13]).c_str() : "on");
+        mexPrintf("  %-20s: %s\n", "normalization (rx)", present(nrhs, prhs, 14) ? name_of(prhs[14]).c_str() : "on");
+        mexPrintf("  %-20s: %s\n", "semantics", p.semantics == FN_SEMANTICS_MEX ? "mex" : "cpu");
+    }
     if (own_tx) mxDestroyArray(Ftx);
     if (own_rx) mxDestroyArray(Frx);
     p.f_number_tx_values = Ft.data();
@@ -138,6 +151,9 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     const size_t nv = (size_t)p.n_x * p.n_z;
     std::vector<float> img(2 * nv), wt(nv), wr(nv);
     check(fn_das_sa(&p, mxGetPr(prhs[2]), FN_DTYPE_F64, img.data(), wt.data(), wr.data()), GW);
+    // device, launch, memory and timing sections (gpu_bf_saft_rf.cu, as gpu_bf_das_pw_rf.cu:664-689, 875-903,
+    // 1128-1144)
+    if (p.verbosity > 0) mexPrintf("%s", fn_last_report());
     plhs[0] = image_out(img, p.n_z, p.n_x);
     if (nlhs > 1) plhs[1] = weights_out(wt, p.n_z, p.n_x);
     if (nlhs > 2) plhs[2] = weights_out(wr, p.n_z, p.n_x);

@@ -183,6 +183,11 @@ typedef struct fn_sa_params {
 int32_t fn_abi_version(void);
 const char *fn_status_id(int32_t status);
 const char *fn_last_error(void);
+/* run-time report of this thread's last fn_das_pw / fn_das_sa call made with
+ * verbosity >= 1 (empty otherwise): device properties, launch parameters, memory
+ * consumption and per-stage times, the sections gpu_bf_das_pw_rf.cu:664-689,
+ * 875-903 and 1128-1144 print; the MEX gateways print it after their status lines */
+const char *fn_last_report(void);
 /* number of HIP devices (gpu_bf_das_pw_rf.cu:115-121) */
 int32_t fn_device_count(int32_t *count);
 
@@ -300,6 +305,18 @@ int32_t fn_batch_execute(fn_batch *batch, const void *const *d_rf, const int32_t
  * Default: disabled (one launch per plan measured faster on the mixed
  * ensemble + job-queue batch since the pipelined tile staging). */
 int32_t fn_batch_set_persistent(fn_batch *batch, int32_t enable);
+/* Row stacks: single-frame plane-wave plans (up to 128 elements, at most 48
+ * transmit rows each) that differ only in their steering angles and RF --
+ * equal voxel grid, sample count, band, F-numbers, window, normalization,
+ * semantics, precision, speed of sound and delay offsets -- are beamformed by
+ * one launch over their stacked transmit rows (16 per matrix-core tile), each
+ * row added to its own plan's image.  The per-(voxel, bin, element) weights
+ * are then generated once for up to 64 rows of several plans.  No counterpart
+ * in the reference (one MEX call per acquisition).  Default: enabled; enable =
+ * 0 beamforms every plan on its own; the next execute captures again.
+ * fn_batch_stacks reports the stacks of the last capture and the plans in them. */
+int32_t fn_batch_set_stacking(fn_batch *batch, int32_t enable);
+int32_t fn_batch_stacks(const fn_batch *batch, int32_t *n_stacks, int32_t *n_stacked_plans);
 /* graph nodes and the number of captures so far */
 int32_t fn_batch_info(const fn_batch *batch, int64_t *n_nodes, int32_t *n_captures);
 int32_t fn_batch_destroy(fn_batch *batch);

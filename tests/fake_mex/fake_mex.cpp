@@ -2,7 +2,8 @@
 // (TEST INFRASTRUCTURE).  Arrays are plain C++ objects; MATLAB objects are an
 // class name plus properties; mexCallMATLAB serves the functions the gateways
 // call back: isa, compute_values (through libfnumber_hip's F-number port, as
-// the classes in +f_numbers would), the default F-number constructor.  Errors
+// the classes in +f_numbers would), the default F-number constructor, and
+// string / char for the names in the status section.  Errors
 // raised with mexErrMsgIdAndTxt unwind to fm_call, which reports the id.
 #include "mex.h"
 
@@ -181,6 +182,28 @@ int mexCallMATLAB(int nlhs, mxArray *plhs[], int nrhs, mxArray *prhs[], const ch
         plhs[0] = r;
         return 0;
     }
+    if (fn == "string" && nrhs == 1 && nlhs == 1) {
+        // the class name and its numeric properties, standing in for the classes' string methods
+        const mxArray *o = prhs[0];
+        std::string t = o->cls;
+        std::string sep = " (";
+        for (const auto &kv : o->props) {
+            if (kv.second->cls != "double" || kv.second->pr.size() != 1) continue;
+            char b[64];
+            std::snprintf(b, sizeof(b), "%s%s = %.2f", sep.c_str(), kv.first.c_str(), kv.second->pr[0]);
+            t += b;
+            sep = ", ";
+        }
+        if (sep == ", ") t += ")";
+        auto *r = mxCreateString(t.c_str());
+        r->cls = "string";
+        plhs[0] = r;
+        return 0;
+    }
+    if (fn == "char" && nrhs == 1 && nlhs == 1 && prhs[0]->cls == "string") {
+        plhs[0] = mxCreateString(prhs[0]->str.c_str());
+        return 0;
+    }
     mexErrMsgIdAndTxt("fake_mex:UnknownFunction", "mexCallMATLAB: %s", name);
     return 1;
 }
@@ -201,6 +224,7 @@ int mexPrintf(const char *fmt, ...)
     va_start(ap, fmt);
     int n = std::vprintf(fmt, ap);
     va_end(ap);
+    std::fflush(stdout);
     return n;
 }
 

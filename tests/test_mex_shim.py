@@ -125,6 +125,64 @@ def test_sa_gateway_matches_python_api(sa):
     assert sa.call(1, bad)[1] == "gpu_bf_saft_rf:InvalidRFData"
 
 
+@pytest.mark.gpu
+def test_gateways_report_at_verbosity_one(pw, sa, capfd):
+    """Verbosity 1 prints the status section before the computation (geometry, band
+    with its indices, names of window, F-number and normalization through MATLAB's
+    string and char, gpu_bf_das_pw_rf.cu:609-632) and, after it, the device, launch,
+    memory and timing sections (gpu_bf_das_pw_rf.cu:664-689, 875-903, 1128-1144);
+    verbosity 0 prints nothing."""
+    import re
+
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.cpwc_small(), 12, 10, n_angles=3, n_samples=1024, n_speckle=300)
+    rf_m = np.transpose(w.rf_host(), (2, 1, 0)).astype(np.float64)
+    args = [pw.double(w.positions_x), pw.double(w.positions_z), pw.double(rf_m), pw.double(w.f_s),
+            pw.double(w.angles), pw.double(w.width), pw.double(w.pitch), pw.double(w.c_0),
+            pw.double(list(w.f_bounds)), pw.double(0.0), pw.window("tukey", fraction_cosine=0.5),
+            pw.fnumber("f_numbers.grating.angle_lb", angle_lb_deg=60.0, F_number_ub=1.5, distance_deg=10.0),
+            pw.normalization("on"), pw.double(0.0)]
+    capfd.readouterr()
+    assert pw.call(1, args + [pw.double(0.0)])[1] is None
+    assert capfd.readouterr().out == ""
+    assert pw.call(1, args + [pw.double(1.0)])[1] is None
+    out = capfd.readouterr().out
+    print(out)
+    for section in ("status information", "lattice geometry", "lower frequency bound", "Selected device 0 of",
+                    "launch parameters", "memory consumption", "performance statistics"):
+        assert section in out, section
+    assert "windows.tukey (fraction_cosine = 0.50)" in out
+    assert "f_numbers.grating.angle_lb (" in out and "angle_lb_deg = 60.00" in out
+    assert re.search(r"lattice geometry\s*:\s*12 x 10", out)
+    assert out.index("status information") < out.index("Selected device")
+    times = {k: float(v) for k, v in re.findall(r"^\s+(memory transfer: host to device|band-limited DFT \(spectra\)|"
+                                                 r"beamforming kernel|memory transfer: device to host|total)\s*:\s*"
+                                                 r"([0-9.]+) ms", out, re.M)}
+    assert len(times) == 5 and times["beamforming kernel"] > 0
+    assert abs(sum(v for k, v in times.items() if k != "total") - times["total"]) < 2e-3 + 1e-3 * times["total"]
+    mem = out[out.index("memory consumption"):out.index("performance statistics")]
+    kib = re.findall(r"^\s+(.+?)\s*:\s+([0-9.]+) KiByte", mem, re.M)
+    parts = [float(v) for k, v in kib if k != "total"]
+    total = float(dict(kib)["total"])
+    assert abs(sum(parts) - total) < 0.05 * len(parts)
+    rf_kib = dict(kib)["RF signals"]
+    assert abs(float(rf_kib) - rf_m.size * 8 / 1024) < 0.01
+    # the synthetic-aperture gateway prints both sides' names
+    ws = synth.scaled(synth.saft_full(), 6, 5, n_samples=512, n_speckle=100, n_elements=16)
+    rs = np.transpose(ws.rf_host(), (2, 1, 0)).astype(np.float64)
+    F = sa.fnumber("f_numbers.grating.angle_lb", angle_lb_deg=60.0, F_number_ub=1.5, distance_deg=10.0)
+    sargs = [sa.double(ws.positions_x), sa.double(ws.positions_z), sa.double(rs), sa.double(ws.f_s),
+             sa.double(ws.width), sa.double(ws.pitch), sa.double(ws.c_0), sa.double(list(ws.f_bounds)),
+             sa.double(0.0), sa.window("hann"), sa.window("boxcar"), F, F, sa.normalization("on"),
+             sa.normalization("off"), sa.double(0.0), sa.double(1.0)]
+    assert sa.call(1, sargs)[1] is None
+    out = capfd.readouterr().out
+    for section in ("window (tx)", "windows.hann", "window (rx)", "windows.boxcar", "normalization (rx)",
+                    "normalizations.off", "Selected device 0 of", "performance statistics"):
+        assert section in out, section
+
+
 @pytest.fixture(scope="module")
 def pwadj():
     mexsim.build()

@@ -455,8 +455,9 @@ def run_batching(steps, n_frames, n_jobs, dev):
     flow phantom) and a queue of n_jobs independent CPWC jobs (grids and angle counts drawn
     with a fixed seed) executed together as one captured graph over 4 streams, every plan
     autotuned; mixed precision (fp16) and the same batch in FP32.  The graph is timed with
-    the jobs of each kernel class as one persistent launch over their work items and with
-    one launch per plan; the faster form is the batch's figure."""
+    one launch per plan, with the jobs of each kernel class as one persistent launch over
+    their work items, and with the jobs that share grid and band as row stacks (one launch
+    over their stacked transmit rows); the fastest form is the batch's figure."""
     import numpy as np
     import torch
 
@@ -511,13 +512,19 @@ def run_batching(steps, n_frames, n_jobs, dev):
     images = {}
     for precision in ("fp16", "fp32"):
         plans, rfs, outs, tunes, t_tune = build(precision)
-        batch = fa.Batch(plans, n_streams=4, persistent=False)
+        batch = fa.Batch(plans, n_streams=4, persistent=False, stacking=False)
         ms_streams = _time_ms(lambda: batch.execute(rfs, outs), steps, 1, dev)
         nodes_streams = batch.info()["graph_nodes"]
         batch.set_persistent(True)  # same-class jobs as one persistent launch over their work items
         ms_pers = _time_ms(lambda: batch.execute(rfs, outs), steps, 1, dev)
-        if ms_streams < ms_pers:
-            batch.set_persistent(False)
+        batch.set_persistent(False)
+        batch.set_stacking(True)  # jobs sharing grid and band as one launch over their stacked rows
+        ms_stack = _time_ms(lambda: batch.execute(rfs, outs), steps, 1, dev)
+        stack_info = batch.info()
+        forms = {"stacked": ms_stack, "persistent": ms_pers, "per_plan": ms_streams}
+        best = min(forms, key=forms.get)
+        batch.set_stacking(best == "stacked")
+        batch.set_persistent(best == "persistent")
         ms = _time_ms(lambda: batch.execute(rfs, outs), steps, 1, dev)
         ens_ms = _time_ms(lambda: plans[0].execute(rfs[0], outs[0]), max(2, steps // 2), 1, dev)
         images[precision] = [o.clone() for o in outs]
@@ -526,7 +533,9 @@ def run_batching(steps, n_frames, n_jobs, dev):
         res[precision] = {"batch_ms": ms, "value": voxel_frames / (ms / 1e3) / 1e6, "unit": "Mvoxel*frames/s",
                           "ensemble_alone_ms": ens_ms, "graph_nodes": info["graph_nodes"],
                           "persistent_ms": ms_pers, "per_plan_launches_ms": ms_streams,
-                          "graph_nodes_per_plan_launches": nodes_streams, "persistent": ms_pers <= ms_streams,
+                          "row_stacked_ms": ms_stack, "row_stacks": stack_info["row_stacks"],
+                          "stacked_plans": stack_info["stacked_plans"], "form": best,
+                          "graph_nodes_per_plan_launches": nodes_streams, "persistent": best == "persistent",
                           "autotune_s": t_tune,
                           "tiling_ensemble": {"m_tiles": tunes[0]["m_tiles"], "bin_split": tunes[0]["bin_split"],
                                               "default_ms": tunes[0]["default_ms"], "best_ms": tunes[0]["best_ms"]},

@@ -258,12 +258,18 @@ class Batch:
     replayed by one graph launch per execute(); a change of any buffer pointer
     captures again."""
 
-    def __init__(self, plans, n_streams: int = 4, persistent: bool = False):
+    def __init__(self, plans, n_streams: int = 4, persistent: bool = False, stacking: bool = True):
         self.plans = list(plans)
         arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
         self._h = C.c_void_p()
         _lib.check(_lib.load().fn_batch_create(arr, len(self.plans), int(n_streams), C.byref(self._h)), "fn_batch")
         self.set_persistent(persistent)
+        self.set_stacking(stacking)
+
+    def set_stacking(self, enable: bool = True):
+        """Plane-wave plans that differ only in steering angles and RF beamformed by one
+        launch over their stacked transmit rows (fnumber_hip.h fn_batch_set_stacking)."""
+        _lib.check(_lib.load().fn_batch_set_stacking(self._h, int(bool(enable))), "fn_batch")
 
     def set_persistent(self, enable: bool = True):
         """Plans of one kernel class as one persistent launch over all their work items
@@ -287,7 +293,10 @@ class Batch:
     def info(self) -> dict:
         nodes, caps = C.c_int64(), C.c_int32()
         _lib.check(_lib.load().fn_batch_info(self._h, C.byref(nodes), C.byref(caps)), "fn_batch")
-        return {"graph_nodes": nodes.value, "captures": caps.value}
+        ns, npl = C.c_int32(), C.c_int32()
+        _lib.check(_lib.load().fn_batch_stacks(self._h, C.byref(ns), C.byref(npl)), "fn_batch")
+        return {"graph_nodes": nodes.value, "captures": caps.value, "row_stacks": ns.value,
+                "stacked_plans": npl.value}
 
     def close(self):
         if self._h:

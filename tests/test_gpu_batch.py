@@ -247,3 +247,58 @@ def test_persistent_batch_matches_eager(fa, torch):
     batch.close()
     for j in jobs:
         j[0].close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp16", 1e-4), ("bf16x3", 1e-5)])
+def test_row_stacks_match_eager(fa, torch, oracle, precision, tol):
+    """Plans that differ only in steering angles and RF (one grid, 1 to 31 angles, angle
+    counts that do not fill a tile) beamformed as one row stack: every plan's image
+    equals its own eager execute on every replay, with eager work in between, and the
+    first plan's the FP64 oracle.  Plans of another grid, another window or more than
+    48 rows stay on their own; with stacking off the batch gives the same images."""
+    specs = [(40, 36, 1, "hann"), (40, 36, 3, "hann"), (40, 36, 11, "hann"), (40, 36, 31, "hann"),
+             (40, 36, 5, "hann"), (40, 36, 7, "hann"), (40, 36, 11, "hann"),
+             (40, 36, 3, "tukey"), (40, 36, 5, "tukey"),  # a second stack (other window)
+             (32, 36, 3, "hann"),                         # another grid: on its own
+             (40, 36, 60, "hann")]                        # four tiles of rows: on its own
+    jobs = []
+    for k, (nx, nz, na, win) in enumerate(specs):
+        w = _cpwc(nx, nz, na, 900 + k)
+        jobs.append((_plan(fa, w, precision=precision, win=win), w, w.rf_device("cuda:0"), (nx, nz)))
+    eager, outs = [], []
+    for plan, w, rf, shape in jobs:
+        e = torch.empty(shape, dtype=torch.complex64, device="cuda")
+        plan.execute(rf, e)
+        eager.append(e)
+        outs.append(torch.empty_like(e))
+    torch.cuda.synchronize()
+    rfs = [j[2] for j in jobs]
+    batch = fa.Batch([j[0] for j in jobs], n_streams=3)
+    for stacking in (True, False):
+        batch.set_stacking(stacking)
+        for replay in range(3):
+            for o in outs:
+                o.fill_(float("nan"))  # eager work between replays
+            batch.execute(rfs, outs)
+            torch.cuda.synchronize()
+            for i, (o, e) in enumerate(zip(outs, eager)):
+                err = (torch.linalg.vector_norm(o - e) / torch.linalg.vector_norm(e)).item()
+                assert err < tol, f"stacking={stacking} replay {replay} job {i}: {err}"
+        info = batch.info()
+        print(f"case {precision} stacking={stacking}: {info}", flush=True)
+        if stacking:
+            assert info["row_stacks"] == 2 and info["stacked_plans"] == 9
+        else:
+            assert info["row_stacks"] == 0
+    w = jobs[2][1]
+    ref, _ = oracle.das_pw(w.positions_x, w.positions_z, w.rf_host(), w.f_s, w.angles, w.width, w.pitch, w.c_0,
+                           w.f_bounds, 0, ("hann", 0.0), oracle.fnumber("angle_lb", 60, 1.5, 10), "on")
+    batch.set_stacking(True)
+    batch.execute(rfs, outs)
+    torch.cuda.synchronize()
+    e = rel_l2(outs[2].cpu().numpy().T, ref)
+    print(f"  stacked plan 2 vs oracle: {e:.3e}", flush=True)
+    assert e <= {"fp32": 1e-4, "fp16": 1e-3, "bf16x3": 1e-4}[precision]
+    batch.close()
+    for j in jobs:
+        j[0].close()

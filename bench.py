@@ -82,6 +82,7 @@ def parse():
     ap.add_argument("--no-batching", action="store_true", help="skip the batching-form measurements")
     ap.add_argument("--no-saft", action="store_true", help="skip the SAFT precision-form measurements")
     ap.add_argument("--no-line-scan", action="store_true", help="skip the line-by-line scanning measurement")
+    ap.add_argument("--no-wide", action="store_true", help="skip the 192 / 256-element arrays")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 11-angle 256 x 256 CPWC arm (the CPU-runnable workload at GPU scale)")
     ap.add_argument("--bin-split", type=int, default=0,
@@ -356,6 +357,57 @@ def run_small(steps, warmup, dev):
             "tiling": {"m_tiles": r["info"]["m_tiles"], "bin_split": r["info"]["bin_split"]},
             "autotuned": {"m_tiles": tune["m_tiles"], "bin_split": tune["bin_split"], "beamform_ms": beam_tuned,
                           "roofline": precision_roofline("fp32", flops, beam_tuned)}}
+
+
+def run_wide(steps, warmup, dev):
+    """Arrays beyond 128 elements: the 11-angle CPWC workload (2048 samples, 256 x 256,
+    Tukey(0.5), angle_lb(60, 1.5)) at 192 and 256 elements, and the full synthetic aperture
+    at 192 and 256 elements (3000 samples, 256 x 256); FP32 and bf16x3, default tilings."""
+    import torch
+
+    import f_number_amd as fa
+    from f_number_amd import synth
+
+    out = {}
+    for n_el in (192, 256):
+        w = synth.scaled(synth.cpwc_small(), 256, 256, n_elements=n_el)
+        w.seed = 20260100 + n_el
+        for prec in ("fp32", "bf16x3"):
+            kw = dict(window=fa.windows.tukey(0.5), F_number=fa.f_numbers.grating.angle_lb(60, 1.5), precision=prec)
+            r = run_arm(w, kw, steps, warmup, dev)
+            flops = r["info"]["flops_per_frame"]
+            out[f"pw_{n_el}el_{prec}"] = {"beamform_ms": r["beamform_ms"], "spectra_ms": r["spectra_ms"],
+                                          "value": w.n_voxels / (r["ms_per_step"] / 1e3) / 1e6,
+                                          "m_tiles": r["info"]["m_tiles"], "bin_split": r["info"]["bin_split"],
+                                          "roofline": precision_roofline(prec, flops, r["beamform_ms"]),
+                                          "finite": r["finite"]}
+        ws = synth.scaled(synth.saft_full(), 256, 256, n_elements=n_el)
+        ws.seed = 20260200 + n_el
+        rf = ws.rf_device(dev)
+        for prec in ("fp32", "bf16x3"):
+            plan = fa.SaPlan(ws.positions_x, ws.positions_z, ws.n_samples, ws.n_elements, ws.f_s, ws.width, ws.pitch,
+                             ws.c_0, ws.f_bounds, 0.0, window_tx=fa.windows.tukey(0.5),
+                             window_rx=fa.windows.tukey(0.5), F_number_tx=fa.f_numbers.grating.angle_lb(60, 1.5),
+                             F_number_rx=fa.f_numbers.grating.angle_lb(60, 1.5), precision=prec,
+                             index_gpu=dev.index or 0)
+            info = plan.info()
+            img = torch.empty((ws.positions_x.size, ws.positions_z.size), dtype=torch.complex64, device=dev)
+            for _ in range(warmup):
+                plan.execute(rf, img)
+            torch.cuda.synchronize(dev)
+            plan.set_profiling(True)
+            plan.get_profile()
+            for _ in range(steps):
+                plan.execute(rf, img)
+            torch.cuda.synchronize(dev)
+            spec_ms, beam_ms, n = plan.get_profile()
+            plan.close()
+            beam_ms /= max(n, 1)
+            out[f"sa_{n_el}el_{prec}"] = {"beamform_ms": beam_ms, "prep_ms": spec_ms / max(n, 1),
+                                          "roofline": precision_roofline(prec, info["flops_per_frame"], beam_ms),
+                                          "finite": bool(torch.isfinite(torch.view_as_real(img)).all().item())}
+        del rf
+    return out
 
 
 def run_diff(steps, warmup, dev):
@@ -670,6 +722,8 @@ def main():
     }
     if not args.no_small:
         out["cpwc_11ang_256x256"] = run_small(args.steps, args.warmup, dev)
+    if not args.no_wide:
+        out["wide_arrays"] = run_wide(max(3, args.steps // 2), 1, dev)
     if not args.no_diff:
         out["differentiation"] = run_diff(max(3, args.steps // 2), 1, dev)
     if not args.no_line_scan:

@@ -890,7 +890,16 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 // read from fixed registers and the first step starts the accumulators
 #pragma unroll
                 for (int jj = 0; jj < SJC; ++jj) step(cc * SJC + jj);
-            } else if constexpr (LBL || MT > 2 || KI > 32) {
+            } else if constexpr (KI > 32 && !LBL) {
+                // wide arrays: every step unrolled under its own uniform guard, so that the
+                // 48 / 64 phase words stay in registers (a runtime index would move them to
+                // scratch memory)
+#pragma unroll
+                for (int jj = 0; jj < SJC; ++jj) {
+                    const int j = cc * SJC + jj;
+                    if (j >= j_a && j <= j_b) step(j);
+                }
+            } else if constexpr (LBL || MT > 2) {
                 // (three or more tiles: a step's 9+ products keep the matrix cores busy)
 #pragma clang loop unroll(disable)
                 for (int j = j_a; j <= j_b; ++j) step(j);

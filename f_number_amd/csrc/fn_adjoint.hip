@@ -61,13 +61,13 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     // (for WC_COS_RF the two inverse half-widths carry the window slope q_s)
     __shared__ __align__(16) uint4 apx[ADJ_WAVES][64];
     const BeamArgs &A = S.B;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int cidx = lane & 15, q = lane >> 4;
     const long n_vox = (long)A.n_x * A.n_z;
-    const int b = blockIdx.y * ADJ_WAVES + wave;
+    const int b = blockIdx.x * ADJ_WAVES + wave;
     const bool active = b < A.n_f;
     const int at = blockIdx.z;
-    const long vr0 = (long)blockIdx.x * S.vox_per_range;
+    const long vr0 = (long)blockIdx.y * S.vox_per_range;
     const long vr1 = min(vr0 + S.vox_per_range, n_vox);
     const float pf = (float)A.pitch;
     const uint32_t n = (uint32_t)(A.n_lb + (active ? b : 0));
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
         }
         r.sc = 0.f;
         if (tid < ADJ_WAVES * ADJ_GV) {
-            const int bs = blockIdx.y * ADJ_WAVES + tid / ADJ_GV;
+            const int bs = blockIdx.x * ADJ_WAVES + tid / ADJ_GV;
             const long u3 = base + tid % ADJ_GV;
             if (bs < A.n_f && u3 < n_vox) r.sc = S.vsc[(size_t)bs * n_vox + u3];
         }
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     // lane holds D[4q + r][cidx] = Z(element 16 mt + 4q + r, angle column cidx) of the range;
     // partial layout [range][bin][angle tile][plane][element 128][angle 16]
     const int per_bin = 2 * ADJ_ET * 256;
-    float *dst = S.P + (((size_t)blockIdx.x * A.n_f + b) * S.n_at + at) * per_bin;
+    float *dst = S.P + (((size_t)blockIdx.y * A.n_f + b) * S.n_at + at) * per_bin;
 #pragma unroll
     for (int mt = 0; mt < ADJ_ET; ++mt)
 #pragma unroll

@@ -1030,7 +1030,9 @@ int contract_adjoint(fn_plan *P, const float *d_y, hipStream_t s)
     S.vsc = P->d_vsc;
     S.vox_per_range = P->adj_vpr;
     S.n_at = P->adj_at;
-    const dim3 grid((unsigned)P->adj_vr, (unsigned)n_chunks, (unsigned)P->adj_at);
+    // bin quads fastest: the workgroups dispatched together share a voxel range, whose
+    // phase-word tables then stay in L2
+    const dim3 grid((unsigned)n_chunks, (unsigned)P->adj_vr, (unsigned)P->adj_at);
     // the phase tables depend on the geometry and the time of flight only: rebuilt
     // after set_tof, or when this call's stream is not the one that built them; the
     // normalization table on the geometry only: built once per stream

@@ -55,11 +55,13 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     // group tables, double-buffered: the next group is loaded while this one is contracted
     __shared__ __align__(16) uint32_t q_rx[2][ADJ_GV * ADJ_QS];  // [voxel i][mt][cidx]: element cidx + 16 mt
     __shared__ uint32_t q_tx[2][ADJ_GV * 16];      // [voxel i][angle column]
-    __shared__ float2 vy[2][ADJ_GV];               // cotangent of the group's voxels (0 outside the range)
-    __shared__ float scv[2][ADJ_WAVES][ADJ_GV];    // s / w_n(v) of each wave's bin for the group's voxels
-    // per wave, for 64 voxels: (klo | khi << 16, 1/W_L, 1/W_R, x_0 - x) with x_0 = -M pitch
-    // (for WC_COS_RF the two inverse half-widths carry the window slope q_s)
+    // cotangent times s / w_n(v) of each wave's bin for the group's voxels (0 outside the range)
+    __shared__ float2 ys[2][ADJ_WAVES][ADJ_GV];
+    // per wave, for 64 voxels: the window argument of element e on each side as e r1 / 16 + r0
+    // and e l1 / 16 + l0 (r1, l1 the slopes of 16 elements; for WC_COS_RF with the window's
+    // quarter-period scale and offset folded in), and the support klo | khi << 16
     __shared__ __align__(16) uint4 apx[ADJ_WAVES][64];
+    __shared__ uint32_t akk[ADJ_WAVES][64];
     const BeamArgs &A = S.B;
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int cidx = lane & 15, q = lane >> 4;
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     const int at = blockIdx.z;
     const long vr0 = (long)blockIdx.y * S.vox_per_range;
     const long vr1 = min(vr0 + S.vox_per_range, n_vox);
-    const float pf = (float)A.pitch;
+    const float c16 = (float)cidx * 0.0625f;  // element cidx in units of 16 elements
     const uint32_t n = (uint32_t)(A.n_lb + (active ? b : 0));
     double Fb = 0.0, sqf = 0.0;
     if (active) {
@@ -94,8 +96,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
     struct GroupRegs {
         uint4 rx[RX4];
         uint32_t tx[TX1];
-        float2 y;
-        float sc;
+        float2 ys;
     };
     static_assert(ADJ_WAVES * ADJ_GV <= 256, "one scale per thread");
     auto fetch_group = [&](long base) {
@@ -113,16 +114,15 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
             const long uv = base + iv;
             r.tx[h] = uv < n_vox ? S.qtx[(size_t)uv * qtx_stride + 16 * at + col] : 0u;
         }
-        r.y = make_float2(0.f, 0.f);
-        if (tid < ADJ_GV) {
-            const long u2 = base + tid;
-            if (u2 < vr1) r.y = S.y[u2];
-        }
-        r.sc = 0.f;
+        r.ys = make_float2(0.f, 0.f);
         if (tid < ADJ_WAVES * ADJ_GV) {
             const int bs = blockIdx.x * ADJ_WAVES + tid / ADJ_GV;
             const long u3 = base + tid % ADJ_GV;
-            if (bs < A.n_f && u3 < n_vox) r.sc = S.vsc[(size_t)bs * n_vox + u3];
+            if (bs < A.n_f && u3 < vr1) {
+                const float sc = S.vsc[(size_t)bs * n_vox + u3];
+                const float2 y = S.y[u3];
+                r.ys = make_float2(sc * y.x, sc * y.y);
+            }
         }
         return r;
     };
@@ -134,8 +134,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
         }
 #pragma unroll
         for (int h = 0; h < TX1; ++h) q_tx[buf][tid + 256 * h] = r.tx[h];
-        if (tid < ADJ_GV) vy[buf][tid] = r.y;
-        if (tid < ADJ_WAVES * ADJ_GV) scv[buf][tid / ADJ_GV][tid % ADJ_GV] = r.sc;
+        if (tid < ADJ_WAVES * ADJ_GV) ys[buf][tid / ADJ_GV][tid % ADJ_GV] = r.ys;
     };
     // per-voxel aperture terms of voxel `base + lane` (clamped into the grid)
     auto load_geo = [&](long base) {
@@ -166,11 +165,15 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
                 double Fr = Fb;
                 if (A.clamp) Fr = fmin(Fr, sqf * geo.w);  // das_pw.m:374-377
                 const Aperture ap = make_aperture(geo.x, geo.y, geo.z, Fr, A, A.support_rx);
-                // WC_COS_RF keeps the quarter-period slope folded in: q_s / W per side
-                const float sl = WCR == WC_COS_RF ? A.wrx.q_s : 1.0f;
-                apx[wave][lane] = make_uint4((uint32_t)ap.klo | ((uint32_t)ap.khi << 16),
-                                             __float_as_uint(ap.inv_wl * sl), __float_as_uint(ap.inv_wr * sl),
-                                             __float_as_uint((float)(x0 - geo.x)));
+                // d = x_e - x = (x_0 - x) + e pitch; right side d / W_R, left side -d / W_L
+                // (WC_COS_RF: times q_s, plus q_o)
+                const double sl = WCR == WC_COS_RF ? (double)A.wrx.q_s : 1.0;
+                const double qo = WCR == WC_COS_RF ? (double)A.wrx.q_o : 0.0;
+                const double wr = (double)ap.inv_wr * sl, wl = (double)ap.inv_wl * sl, dx = x0 - geo.x;
+                akk[wave][lane] = (uint32_t)ap.klo | ((uint32_t)ap.khi << 16);
+                apx[wave][lane] = make_uint4(__float_as_uint((float)(16.0 * A.pitch * wr)),
+                                             __float_as_uint((float)(-16.0 * A.pitch * wl)),
+                                             __float_as_uint((float)(dx * wr + qo)), __float_as_uint((float)(qo - dx * wl)));
                 if (base + 64 < vr1) geo = load_geo(base + 64);  // consumed four groups later
             }
 
@@ -178,7 +181,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
             // group (wave-uniform): every other tile has zero window for all 16 voxels
             int t_lo, t_hi;
             {
-                const uint32_t kk = apx[wave][16 * gq + cidx].x;
+                const uint32_t kk = akk[wave][16 * gq + cidx];
                 const int kl = (int)(kk & 0xffffu), kh = (int)(kk >> 16);
                 const int lo = row_min16(kl <= kh ? kl : 0x7fff), hi = row_max16(kl <= kh ? kh : -1);
                 t_lo = __builtin_amdgcn_readfirstlane(lo) >> 4;
@@ -188,8 +191,8 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
             // Gauss: conj(B) U = (br - j bi)(ur + j ui):
             //   K1 = (br - bi) ur, K2 = br (ui - ur), K3 = bi (ur + ui); Re = K1 + K3, Im = K1 + K2
             // Per voxel step s (voxel 4 s + q of the group): U, and the window argument as
-            // max(d r1 + r0, d l1 + l0) over the element tile mt, d = x_k - x = dg + 16 mt pitch
-            // (each side's term is the larger exactly on its own side, so no per-element select)
+            // max(mt R1 + R0, mt L1 + L0) over the element tile mt (each side's term is the
+            // larger exactly on its own side, so no per-element select)
             float Ur[4], U2[4], U3[4], R0[4], R1[4], L0[4], L1[4];
             int Klo[4], Khi[4];
             int qoff[4];
@@ -197,23 +200,21 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
             for (int s = 0; s < 4; ++s) {
                 const int i = 4 * s + q;  // voxel of this lane in step s
                 const uint4 apv = apx[wave][16 * gq + i];  // written by this wave (LDS operations of a wave are ordered)
-                Klo[s] = (int)(apv.x & 0xffffu);
-                Khi[s] = (int)(apv.x >> 16);
-                const float iwl = __uint_as_float(apv.y), iwr = __uint_as_float(apv.z);
-                const float dg = fmaf((float)cidx, pf, __uint_as_float(apv.w));  // x_cidx - x
-                const float qo = WCR == WC_COS_RF ? A.wrx.q_o : 0.f;
-                R0[s] = fmaf(dg, iwr, qo);
-                R1[s] = 16.f * pf * iwr;
-                L0[s] = fmaf(-dg, iwl, qo);
-                L1[s] = -16.f * pf * iwl;
-                // s / w_n(v) of voxel i (0 outside a valid aperture), per-plan table
-                const float sc = a_ok ? scv[cur][wave][o + i] : 0.f;
-                // U = s y conj(T) / w
-                const float2 yv = vy[cur][o + i];
+                if (!FULL) {
+                    const uint32_t kk = akk[wave][16 * gq + i];
+                    Klo[s] = (int)(kk & 0xffffu);
+                    Khi[s] = (int)(kk >> 16);
+                }
+                R1[s] = __uint_as_float(apv.x);
+                L1[s] = __uint_as_float(apv.y);
+                R0[s] = fmaf(c16, R1[s], __uint_as_float(apv.z));  // element cidx of tile 0
+                L0[s] = fmaf(c16, L1[s], __uint_as_float(apv.w));
+                // U = s y conj(T) / w, s / w_n(v) from the per-plan table (0 outside a valid aperture)
+                const float2 yv = a_ok ? ys[cur][wave][o + i] : make_float2(0.f, 0.f);
                 const float rt = rev_of(n * q_tx[cur][(o + i) * 16 + cidx]);
                 const float st = __builtin_amdgcn_sinf(rt), ct = __builtin_amdgcn_cosf(rt);
-                const float ur = sc * (yv.x * ct + yv.y * st);
-                const float ui = sc * (yv.y * ct - yv.x * st);
+                const float ur = fmaf(yv.x, ct, yv.y * st);
+                const float ui = fmaf(yv.y, ct, -(yv.x * st));
                 Ur[s] = ur;
                 U2[s] = ui - ur;
                 U3[s] = ur + ui;
@@ -331,9 +332,10 @@ __global__ __launch_bounds__(256) void adjoint_reduce_kernel(const float *__rest
 //   B[(b, 0)][ch] = Re Z[b][ch],  B[(b, 1)][ch] = Im Z[b][ch].
 // v_mfma_f32_16x16x4f32: lane l supplies A[t0 + l % 16][k0 + l / 16] and
 // B[k0 + l / 16][c0 + l % 16], so one K step of 4 covers two bins.  A block of 4 waves
-// owns 128 samples x 64 channels; a wave 32 samples (2 tiles) x 64 channels (4 tiles).
+// owns 64 samples x 64 channels; a wave 16 samples (1 tile) x 64 channels (4 tiles), with
+// the next step's spectra loaded while the current step's products run.
 // Twiddles in LDS; each lane advances its index m by 2t mod N per step in integers.
-constexpr int IDFT_WT = 2, IDFT_WC = 4;
+constexpr int IDFT_WT = 1, IDFT_WC = 4;
 
 __global__ __launch_bounds__(256) void adjoint_idft_kernel(const float2 *__restrict__ Z, const float2 *__restrict__ tw,
                                                            float *__restrict__ out, int n_t, int n_ch, int N, int n_lb,
@@ -345,7 +347,7 @@ __global__ __launch_bounds__(256) void adjoint_idft_kernel(const float2 *__restr
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, q = lane >> 4;  // q = k within the step: bin q / 2, part q % 2
     const int part = q & 1, db = q >> 1;
-    const int t0 = blockIdx.x * 128 + wave * 32;
+    const int t0 = blockIdx.x * (64 * IDFT_WT) + wave * (16 * IDFT_WT);
     const int c0 = blockIdx.y * 64;
     int m[IDFT_WT], dm[IDFT_WT];
     bool tok[IDFT_WT];
@@ -364,21 +366,27 @@ __global__ __launch_bounds__(256) void adjoint_idft_kernel(const float2 *__restr
     for (int i = 0; i < IDFT_WT; ++i)
 #pragma unroll
         for (int j = 0; j < IDFT_WC; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    auto load_z = [&](int b, float *bv) {
+#pragma unroll
+        for (int j = 0; j < IDFT_WC; ++j) {
+            float2 z = make_float2(0.f, 0.f);
+            if (b < n_f && ch[j] < n_ch) z = Z[(size_t)b * n_ch + ch[j]];
+            bv[j] = part ? z.y : z.x;
+        }
+    };
+    float bn[IDFT_WC];
+    load_z(db, bn);
     for (int b0 = 0; b0 < n_f; b0 += 2) {
-        const int b = b0 + db;
         float a[IDFT_WT], bv[IDFT_WC];
+#pragma unroll
+        for (int j = 0; j < IDFT_WC; ++j) bv[j] = bn[j];
+        load_z(b0 + 2 + db, bn);  // next step's spectra (zeros past the band)
 #pragma unroll
         for (int i = 0; i < IDFT_WT; ++i) {
             const float2 w = tws[m[i]];
             a[i] = tok[i] ? (part ? -w.y : w.x) : 0.f;
             m[i] += dm[i];
             if (m[i] >= N) m[i] -= N;
-        }
-#pragma unroll
-        for (int j = 0; j < IDFT_WC; ++j) {
-            float2 z = make_float2(0.f, 0.f);
-            if (b < n_f && ch[j] < n_ch) z = Z[(size_t)b * n_ch + ch[j]];
-            bv[j] = part ? z.y : z.x;
         }
 #pragma unroll
         for (int i = 0; i < IDFT_WT; ++i)
@@ -567,7 +575,7 @@ hipError_t launch_adjoint_idft(const float2 *Z, const float2 *tw, float *out, in
         if (e != hipSuccess) return e;
         lds_done = lds;
     }
-    hipLaunchKernelGGL(adjoint_idft_kernel, dim3((unsigned)((n_t + 127) / 128), (unsigned)((n_ch + 63) / 64)),
+    hipLaunchKernelGGL(adjoint_idft_kernel, dim3((unsigned)((n_t + 64 * IDFT_WT - 1) / (64 * IDFT_WT)), (unsigned)((n_ch + 63) / 64)),
                        dim3(256), lds, s, Z, tw, out, n_t, n_ch, N, n_lb, n_f);
     return hipGetLastError();
 }

@@ -192,6 +192,7 @@ static hipError_t launch_wct(int wct, dim3 grid, size_t lds, hipStream_t s, cons
     switch (wct) {
     case WC_RANGE: return launch_one<SA_MT, WCR, WC_RANGE>(grid, lds, s, A);
     case WC_LINEAR: return launch_one<SA_MT, WCR, WC_LINEAR>(grid, lds, s, A);
+    case WC_COS_RF: return launch_one<SA_MT, WCR, WC_COS_RF>(grid, lds, s, A);
     default: return launch_one<SA_MT, WCR, WC_COS>(grid, lds, s, A);
     }
 }

@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include <type_traits>
 
 namespace fnk {
@@ -224,11 +226,22 @@ __device__ __forceinline__ int row_max16(int v)
 // aperture of a window cut at t = 1 when pitch > width / 2 (elements outside a
 // valid snapped aperture have t - 1 >= (pitch - width/2) / (h + width/2) > 0;
 // the 1e-3 margin keeps that far above float rounding).
-inline bool range_free(const BeamArgs &A, int n_full)
+inline bool range_free_side(const BeamArgs &A, int support, const WinCoef &w, int n_full)
 {
     if (A.n_el != n_full) return false;
-    if (A.support_rx == SUPPORT_ALL) return true;
-    return A.support_rx == SUPPORT_SNAPPED && A.wrx.t_cut <= 1.0f && 0.5 * A.width <= 0.999 * A.pitch;
+    if (support == SUPPORT_ALL) return true;
+    return support == SUPPORT_SNAPPED && w.t_cut <= 1.0f && 0.5 * A.width <= 0.999 * A.pitch;
+}
+
+inline bool range_free(const BeamArgs &A, int n_full) { return range_free_side(A, A.support_rx, A.wrx, n_full); }
+
+// a cosine window cut at t = 1 whose support test the cut implies (WC_COS_RF):
+// raised cosine (A = B = 1/2) with the flat part ending where the taper starts
+inline bool cos_rf_window(const BeamArgs &A, int support, const WinCoef &w, int n_full)
+{
+    return range_free_side(A, support, w, n_full) && w.t_cut == 1.f && w.A == 0.5f && w.B == 0.5f &&
+           (w.t_flat == w.t_off || (w.t_flat < 0.f && w.t_off == 0.f)) && w.q_s > 0.f && std::isfinite(w.q_s) &&
+           std::isfinite(w.q_o);
 }
 
 // WC_COS_RF with the quarter-period slope folded into the inverse half-width:
@@ -427,13 +440,19 @@ template <int PREC>
 __device__ __forceinline__ void pack8(const float *v, u32x4 &hi, u32x4 &lo)
 {
     if (PREC == PREC_BF16X3) {
-        bf16x8 h, l;
+        // high parts rounded in pairs (one packed conversion), read back as floats from
+        // the packed word (a bf16 is the upper half of its float), low parts from the
+        // residuals
+        bf16x8 l;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            h[s] = (__bf16)v[s];
-            l[s] = (__bf16)(v[s] - (float)h[s]);
+        for (int s = 0; s < 8; s += 2) {
+            typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+            const bf16x2 hp = bf16x2{(__bf16)v[s], (__bf16)v[s + 1]};
+            const uint32_t hw = __builtin_bit_cast(uint32_t, hp);
+            hi[s / 2] = hw;
+            l[s] = (__bf16)(v[s] - __uint_as_float(hw << 16));
+            l[s + 1] = (__bf16)(v[s + 1] - __uint_as_float(hw & 0xffff0000u));
         }
-        hi = __builtin_bit_cast(u32x4, h);
         lo = __builtin_bit_cast(u32x4, l);
     } else {
         f16x8 h, l;
@@ -1092,15 +1111,30 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             const Aperture at = make_aperture(x, xp, zp, Ft, A, A.support_tx);
             float wtx[MT][4];
             float tsum = 0.f;
+            // WC_COS_RF: cos^2 of the quarter-period argument max(k r1 + r0, k l1 + l0) for
+            // transmitter offset k = 16 t + r from this lane's first row (each side's term
+            // is the larger exactly on its own side); the window's cut is the support test
+            const float sr = at.inv_wr * A.wtx.q_s, sl = at.inv_wl * A.wtx.q_s;
+            const float tr1 = pf * sr, tl1 = -pf * sl;
+            const float tr0 = fmaf(dg_tx, sr, A.wtx.q_o), tl0 = fmaf(-dg_tx, sl, A.wtx.q_o);
+            auto tx_weight = [&](int t, int r) __attribute__((always_inline)) {
+                const int i = 16 * t + 4 * g + r;
+                const float kf = (float)(16 * t + r);
+                if (WCT == WC_COS_RF) {
+                    const float c = __builtin_amdgcn_cosf(
+                        __builtin_amdgcn_fmed3f(fmaxf(fmaf(kf, tr1, tr0), fmaf(kf, tl1, tl0)), 0.f, 0.25f));
+                    return c * c;
+                }
+                const float d = fmaf(kf, pf, dg_tx);
+                const float tt = fabsf(d) * (d < 0.f ? at.inv_wl : at.inv_wr);
+                const float w = window_eval<WCT>(tt, A.wtx);
+                return (i >= at.klo && i <= at.khi) ? w : 0.f;
+            };
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int i = 16 * (mt0 + mt) + 4 * g + r;
-                    const float d = fmaf((float)(16 * (mt0 + mt) + r), pf, dg_tx);
-                    const float t = fabsf(d) * (d < 0.f ? at.inv_wl : at.inv_wr);
-                    float w = window_eval<WCT>(t, A.wtx);
-                    w = (i >= at.klo && i <= at.khi) ? w : 0.f;
+                    const float w = tx_weight(mt0 + mt, r);
                     wtx[mt][r] = w;
                     tsum += w;
                 }
@@ -1111,13 +1145,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 tsum = 0.f;
                 for (int t = 0; t < MT_all; ++t)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int i = 16 * t + 4 * g + r;
-                        const float d = fmaf((float)(16 * t + r), pf, dg_tx);
-                        const float tt = fabsf(d) * (d < 0.f ? at.inv_wl : at.inv_wr);
-                        const float w = window_eval<WCT>(tt, A.wtx);
-                        tsum += (i >= at.klo && i <= at.khi) ? w : 0.f;
-                    }
+                    for (int r = 0; r < 4; ++r) tsum += tx_weight(t, r);
             }
             tsum += __shfl_xor(tsum, 16);
             tsum += __shfl_xor(tsum, 32);

@@ -317,15 +317,21 @@ int setup_windowed(fn_plan *P, bool rx, const fn_window &wrx, const fn_window &n
 // launch the general classes
 void use_cos_all(fn_plan *P)
 {
+    if (P->nw) return;
     const fnk::WinCoef &w = P->args.wrx;
-    if (P->wcr != fnk::WC_COS || P->nw) return;
-    if (P->args.support_rx == fnk::SUPPORT_ALL && P->n_el == 4 * P->KI && w.t_off == 0.f && w.t_flat < 0.f &&
-        std::isinf(w.t_cut))
-        P->wcr = fnk::WC_COS_ALL;
-    else if (fnk::range_free(P->args, 4 * P->KI) && w.t_cut == 1.f && w.A == 0.5f && w.B == 0.5f &&
-             (w.t_flat == w.t_off || (w.t_flat < 0.f && w.t_off == 0.f)) && w.q_s > 0.f &&
-             std::isfinite(w.q_s) && std::isfinite(w.q_o))
-        P->wcr = fnk::WC_COS_RF;
+    if (P->wcr == fnk::WC_COS) {
+        if (P->args.support_rx == fnk::SUPPORT_ALL && P->n_el == 4 * P->KI && w.t_off == 0.f && w.t_flat < 0.f &&
+            std::isinf(w.t_cut))
+            P->wcr = fnk::WC_COS_ALL;
+        else if (fnk::cos_rf_window(P->args, P->args.support_rx, w, 4 * P->KI))
+            P->wcr = fnk::WC_COS_RF;
+    }
+    // the synthetic aperture's transmit side: the same cut-window class for its
+    // transmitters (the SA kernels launch it; every other SA path evaluates the window
+    // with its support test, which WC_COS_RF maps to)
+    if (P->mode == 1 && P->wct == fnk::WC_COS &&
+        fnk::cos_rf_window(P->args, P->args.support_tx, P->args.wtx, 4 * P->KI))
+        P->wct = fnk::WC_COS_RF;
 }
 
 int wcr_general(int wcr) { return (wcr == fnk::WC_COS_ALL || wcr == fnk::WC_COS_RF) ? fnk::WC_COS : wcr; }
@@ -529,26 +535,55 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
     return ensure_partials(P);
 }
 
-// useful (valid voxel, bin) count of the receive aperture, for the roofline
-double useful_fraction(const fnh::Grid &g, const double *px, int nx, const double *pz, int nz, const double *F,
-                       double f_s, double c0, double pitch, int n_el, bool clamp)
+// Elements whose weight can be nonzero for one voxel and bin: the aperture of
+// das_pw.m:387-411 (das_sa.m:384-400) -- the open boxcar support, the snapped
+// [lb, ub] of a window that vanishes beyond it, or every element of an untruncated
+// window on a non-empty aperture; 0 where the aperture is empty (the bin is skipped).
+int active_elements(double x, double z, double F, int support, float cut, double pitch, int n_el)
 {
     const double M = (n_el - 1) / 2.0;
-    long valid = 0, total = 0;
+    const double hp = z / (2.0 * pitch * F);  // h / pitch (F = 0: the whole array)
+    const double ua = M + (x / pitch - hp), ub = M + (x / pitch + hp);
+    const double lb = std::fmax(std::ceil(ua), 0.0), ubd = std::fmin(std::floor(ub), n_el - 1.0);
+    if (lb > ubd) return 0;
+    if (support == fnk::SUPPORT_OPEN_H) {
+        const double lo = std::fmax(std::floor(ua) + 1.0, 0.0), hi = std::fmin(std::ceil(ub) - 1.0, n_el - 1.0);
+        return hi >= lo ? (int)(hi - lo) + 1 : 0;
+    }
+    if (std::isinf(cut)) return n_el;
+    return (int)(ubd - lb) + 1;
+}
+
+// Useful complex multiply-adds per frame for the roofline (8 flop each): the sum over
+// voxels and bins of the active receive elements times the transmit rows (plane
+// waves), or times the active transmitters (synthetic aperture), from a sample of
+// about 64 x 64 voxels and every bin, scaled to the grid.
+double useful_macs(const fn_plan *P, const double *px, int nx, const double *pz, int nz, const double *Frx,
+                   const double *Ftx, double f_s, double c0, double pitch)
+{
+    const fnh::Grid &g = P->g;
     const int sx = std::max(1, nx / 64), sz = std::max(1, nz / 64);
+    double sum = 0.0;
+    long n = 0;
     for (int ix = 0; ix < nx; ix += sx)
-        for (int iz = 0; iz < nz; iz += sz)
+        for (int iz = 0; iz < nz; iz += sz, ++n)
             for (int b = 0; b < g.n_f; ++b) {
-                double Fv = F[b];
-                const double fn = f_s * (double)(g.n_lb + b) / g.N;
-                if (clamp) Fv = std::fmin(Fv, std::sqrt(fn * pz[iz] / (2.88 * c0)));
-                const double h = pz[iz] / (2.0 * Fv);
-                const double lo = std::fmax(std::ceil(M + (px[ix] - h) / pitch), 0.0);
-                const double hi = std::fmin(std::floor(M + (px[ix] + h) / pitch), n_el - 1.0);
-                valid += lo <= hi;
-                ++total;
+                double Fr = Frx[b];
+                if (P->args.clamp) {
+                    const double fn = f_s * (double)(g.n_lb + b) / g.N;
+                    Fr = std::fmin(Fr, std::sqrt(fn * pz[iz] / (2.88 * c0)));  // das_pw.m:374-377
+                }
+                const int k = active_elements(px[ix], pz[iz], Fr, P->args.support_rx, P->args.wrx.t_cut, pitch, P->n_el);
+                if (!k) continue;
+                if (P->mode == 1) {
+                    const int t = active_elements(px[ix], pz[iz], Ftx[b], P->args.support_tx, P->args.wtx.t_cut, pitch,
+                                                  P->n_el);
+                    sum += (double)k * t;
+                } else {
+                    sum += (double)k * (P->lbl ? 1 : P->n_tx);  // a line scan's voxel meets one row
+                }
             }
-    return total ? (double)valid / total : 0.0;
+    return n ? sum / n * (double)P->n_vox : 0.0;
 }
 
 hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipStream_t s)
@@ -772,8 +807,8 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
     if (P->nw && (s = setup_windowed(P, true, p->window, p->normalization_window, false, p->window,
                                      p->normalization_window))) { destroy(P); return s; }
     use_cos_all(P);
-    P->useful_flops = P->flops * useful_fraction(g, p->positions_x, p->n_x, p->positions_z, p->n_z, p->f_number_values,
-                                                 p->f_s, p->c_0, p->element_pitch, p->n_elements, P->args.clamp != 0);
+    P->useful_flops = 8.0 * useful_macs(P, p->positions_x, p->n_x, p->positions_z, p->n_z, p->f_number_values,
+                                        nullptr, p->f_s, p->c_0, p->element_pitch);
     *out = P;
     return FN_OK;
 }
@@ -802,9 +837,8 @@ int sa_plan(const fn_sa_params *p, fn_plan **out)
              p->f_s, p->element_width, p->element_pitch, p->c_0, p->index_t0,
              p->semantics, p->precision, p->output_scale, p->index_gpu, p->verbosity};
     if ((s = create_common(P, c, g, p->f_number_rx_values, p->f_number_tx_values))) { destroy(P); return s; }
-    P->useful_flops = P->flops * useful_fraction(g, p->positions_x, p->n_x, p->positions_z, p->n_z,
-                                                 p->f_number_rx_values, p->f_s, p->c_0, p->element_pitch,
-                                                 p->n_elements, false);
+    P->useful_flops = 8.0 * useful_macs(P, p->positions_x, p->n_x, p->positions_z, p->n_z, p->f_number_rx_values,
+                                        p->f_number_tx_values, p->f_s, p->c_0, p->element_pitch);
     if (P->nw && (s = setup_windowed(P, nw_rx, p->window_rx, p->normalization_window_rx, nw_tx, p->window_tx,
                                      p->normalization_window_tx))) { destroy(P); return s; }
     use_cos_all(P);

@@ -254,7 +254,9 @@ typedef struct fn_plan_info {
     int64_t n_voxels;
     int64_t workspace_bytes;
     double flops_per_frame;       /* 8 * n_f * n_voxels * n_elements * n_tx */
-    double useful_flops_per_frame;/* same, counting only valid (voxel, bin) pairs */
+    double useful_flops_per_frame;/* same over the elements whose weight can be nonzero (the aperture
+                                     bounds; every element of an untruncated window) and, for the
+                                     synthetic aperture, the active transmitters */
     int32_t tile_voxels, bin_split, m_tiles, k_steps;
 } fn_plan_info;
 int32_t fn_plan_pw_create(const fn_pw_params *p, fn_plan **plan);

@@ -536,10 +536,11 @@ int create_common(fn_plan *P, const Common &c, const fnh::Grid &g, const double 
 }
 
 // Elements whose weight can be nonzero for one voxel and bin: the aperture of
-// das_pw.m:387-411 (das_sa.m:384-400) -- the open boxcar support, the snapped
+// das_pw.m:387-411 (das_sa.m:384-400) -- the open boxcar support (CPU), the snapped
 // [lb, ub] of a window that vanishes beyond it, or every element of an untruncated
-// window on a non-empty aperture; 0 where the aperture is empty (the bin is skipped).
-int active_elements(double x, double z, double F, int support, float cut, double pitch, int n_el)
+// window (CPU Hann and triangular) on a non-empty aperture; 0 where the aperture is
+// empty (the bin is skipped).
+int active_elements(double x, double z, double F, int support, double pitch, int n_el)
 {
     const double M = (n_el - 1) / 2.0;
     const double hp = z / (2.0 * pitch * F);  // h / pitch (F = 0: the whole array)
@@ -550,7 +551,7 @@ int active_elements(double x, double z, double F, int support, float cut, double
         const double lo = std::fmax(std::floor(ua) + 1.0, 0.0), hi = std::fmin(std::ceil(ub) - 1.0, n_el - 1.0);
         return hi >= lo ? (int)(hi - lo) + 1 : 0;
     }
-    if (std::isinf(cut)) return n_el;
+    if (support == fnk::SUPPORT_ALL) return n_el;
     return (int)(ubd - lb) + 1;
 }
 
@@ -573,11 +574,10 @@ double useful_macs(const fn_plan *P, const double *px, int nx, const double *pz,
                     const double fn = f_s * (double)(g.n_lb + b) / g.N;
                     Fr = std::fmin(Fr, std::sqrt(fn * pz[iz] / (2.88 * c0)));  // das_pw.m:374-377
                 }
-                const int k = active_elements(px[ix], pz[iz], Fr, P->args.support_rx, P->args.wrx.t_cut, pitch, P->n_el);
+                const int k = active_elements(px[ix], pz[iz], Fr, P->args.support_rx, pitch, P->n_el);
                 if (!k) continue;
                 if (P->mode == 1) {
-                    const int t = active_elements(px[ix], pz[iz], Ftx[b], P->args.support_tx, P->args.wtx.t_cut, pitch,
-                                                  P->n_el);
+                    const int t = active_elements(px[ix], pz[iz], Ftx[b], P->args.support_tx, pitch, P->n_el);
                     sum += (double)k * t;
                 } else {
                     sum += (double)k * (P->lbl ? 1 : P->n_tx);  // a line scan's voxel meets one row

@@ -159,6 +159,14 @@ hipError_t FN_ENS_LAUNCHER(INST_KI, INST_PREC)(int mt, int wcr, dim3 grid, size_
 #if INST_PREC == 0 || INST_PREC == 3  // FP32, FP16; the split forms stop at 3 tiles (register file)
     case 4: return launch_ens_mt<4>(wcr, grid, lds, s, A);
 #endif
+#if INST_PREC == 3 && INST_KI == 32
+    // fp16 at 128 elements: wider groups, one wave per SIMD (offered to the autotuner,
+    // not the default; at 64 elements 7 and 8 tiles would spill)
+    case 5: return launch_ens_mt<5>(wcr, grid, lds, s, A);
+    case 6: return launch_ens_mt<6>(wcr, grid, lds, s, A);
+    case 7: return launch_ens_mt<7>(wcr, grid, lds, s, A);
+    case 8: return launch_ens_mt<8>(wcr, grid, lds, s, A);
+#endif
     default: return hipErrorInvalidValue;
     }
 }

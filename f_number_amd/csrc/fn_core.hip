@@ -336,12 +336,13 @@ void use_cos_all(fn_plan *P)
 
 int wcr_general(int wcr) { return (wcr == fnk::WC_COS_ALL || wcr == fnk::WC_COS_RF) ? fnk::WC_COS : wcr; }
 
-// transmit tiles per plane-wave workgroup that fit the register file without
-// spills: ensembles hold per-row image accumulators, the split 16-bit forms
-// their residual operands on top of that
-int tile_cap(int n_frames, int precision)
+// default transmit tiles per plane-wave workgroup that fit the register file
+// without spills: ensembles hold per-row image accumulators, the split 16-bit
+// forms their residual operands on top of that; fp16 at 128 elements has room for 5
+int tile_cap(int n_frames, int precision, int KI)
 {
     if (n_frames <= 1) return 5;
+    if (precision == FN_PRECISION_FP16 && KI == 32) return 5;
     return (precision == FN_PRECISION_BF16X3 || precision == FN_PRECISION_FP16X3) ? 3 : 4;
 }
 
@@ -425,8 +426,8 @@ int configure_common(fn_plan *P, const Common &c, const fnh::Grid &g)
         P->n_groups = c.n_x;
     } else if (P->mode == 0) {
         // at most 5 transmit tiles per group keep the LDS tile within 80 KiB
-        // (ensembles: 4, for the per-row image accumulators)
-        const int max_tiles = P->nw ? 2 : std::min(tile_cap(P->n_frames, c.precision), pw_mt_wide(P->KI));
+        // (ensembles: 3-5, for the per-row image accumulators)
+        const int max_tiles = P->nw ? 2 : std::min(tile_cap(P->n_frames, c.precision, P->KI), pw_mt_wide(P->KI));
         const int groups = (P->MT_all + max_tiles - 1) / max_tiles;
         P->MT_tmpl = (P->MT_all + groups - 1) / groups;
         P->n_groups = (P->MT_all + P->MT_tmpl - 1) / P->MT_tmpl;
@@ -850,11 +851,15 @@ size_t dtype_size(int dtype) { return dtype == FN_DTYPE_F64 ? 8 : 4; }
 
 // ---- tiling and autotuning ---------------------------------------------
 
+// transmit tiles per group the autotuner may try: the default cap, and for fp16
+// ensembles at 128 elements up to 8 (one wave per SIMD; each weight serves 128 rows)
 int max_tiles(const fn_plan *P)
 {
     if (P->mode == 1) return sa_mt(P->KI);
     if (P->lbl) return 1;
-    return std::min(P->MT_all, P->nw ? 2 : std::min(tile_cap(P->n_frames, P->precision), pw_mt_wide(P->KI)));
+    if (P->nw) return std::min(P->MT_all, 2);
+    if (P->n_frames > 1 && P->precision == FN_PRECISION_FP16 && P->KI == 32) return std::min(P->MT_all, 8);
+    return std::min(P->MT_all, std::min(tile_cap(P->n_frames, P->precision, P->KI), pw_mt_wide(P->KI)));
 }
 
 int set_tiling(fn_plan *P, int mt, int split)
@@ -1819,7 +1824,7 @@ static int batch_plan_stacks(fn_batch *b, float *const *d_images, std::vector<ch
         }
         const fn_plan *P0 = b->plans[i];
         r.MT_all = (r.rows + 15) / 16;
-        const int max_tiles = tile_cap(2, P0->precision);  // the ensemble kernel's register budget
+        const int max_tiles = tile_cap(2, P0->precision, P0->KI);  // the ensemble kernel's register budget
         const int groups = (r.MT_all + max_tiles - 1) / max_tiles;
         r.MT_tmpl = (r.MT_all + groups - 1) / groups;
         r.n_groups = (r.MT_all + r.MT_tmpl - 1) / r.MT_tmpl;

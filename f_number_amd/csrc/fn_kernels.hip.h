@@ -191,6 +191,57 @@ __device__ __forceinline__ Aperture make_aperture(double x, double xp, double zp
     return ap;
 }
 
+// An aperture in three words, for moving it between lanes: klo | valid << 15 | khi << 16
+// (0 <= klo, khi < 256), then the two reciprocal half-widths.
+struct ApWord {
+    uint32_t kk;
+    float iwl, iwr;
+};
+
+__device__ __forceinline__ ApWord pack_aperture(const Aperture &a)
+{
+    return ApWord{(uint32_t)a.klo | ((uint32_t)a.valid << 15) | ((uint32_t)a.khi << 16), a.inv_wl, a.inv_wr};
+}
+
+__device__ __forceinline__ Aperture unpack_aperture(const ApWord &w)
+{
+    Aperture a;
+    a.klo = (int)(w.kk & 0x7fffu);
+    a.khi = (int)(w.kk >> 16);
+    a.valid = ((w.kk >> 15) & 1u) != 0u;
+    a.inv_wl = w.iwl;
+    a.inv_wr = w.iwr;
+    return a;
+}
+
+// row r (16 lanes, wave-uniform r) of x in every row: v_permlane32_swap with itself
+// gives [x0 x1 x0 x1] and [x2 x3 x2 x3] (rows), v_permlane16_swap of the chosen one
+// with itself [y0 y0 y2 y2] and [y1 y1 y3 y3]
+__device__ __forceinline__ uint32_t row_broadcast(uint32_t x, int r)
+{
+    const auto s = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    const uint32_t y = (r & 2) ? s[1] : s[0];
+    const auto p = __builtin_amdgcn_permlane16_swap(y, y, false, false);
+    return (r & 1) ? p[1] : p[0];
+}
+
+// the aperture word of row r (lanes 16 r + c) in every row, for lane c of the row:
+// on the vector ALU (two lane swaps per word) or through the LDS crossbar (ds_bpermute,
+// no LDS storage)
+template <bool SWAP>
+__device__ __forceinline__ ApWord row_aperture(const ApWord &w, int r, int c)
+{
+    if constexpr (SWAP) {
+        return ApWord{row_broadcast(w.kk, r), __uint_as_float(row_broadcast(__float_as_uint(w.iwl), r)),
+                      __uint_as_float(row_broadcast(__float_as_uint(w.iwr), r))};
+    } else {
+        const int src = ((r << 4) | c) << 2;
+        return ApWord{(uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)w.kk),
+                      __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(w.iwl))),
+                      __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(w.iwr)))};
+    }
+}
+
 // sum over the 16 lanes of a DPP row, result in every lane of the row:
 // quad xor 1, quad xor 2, half-row mirror (quad <-> quad), row mirror (half <-> half)
 __device__ __forceinline__ float row_sum16(float v)
@@ -751,8 +802,32 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             issue_chunk(std::integral_constant<int, (cc + STAGE_NC - 1) % STAGE_NC>{}, b + (cc + STAGE_NC - 1) / STAGE_NC);
     };
 
+    // Receive apertures (FP64 bounds, a division and two reciprocals per voxel and
+    // bin).  The four lane groups g hold the same 16 voxels, so for plane waves group g
+    // computes the aperture of bin b0 + g once per four bins and each bin takes its own
+    // from group (b - b0) & 3: bitwise the same apertures.  The FP32 instances of three
+    // or more tiles move it by lane swaps, the others by a lane permute (each measured
+    // the faster for them).  (The synthetic-aperture instances have no registers to
+    // spare for it.)
+    constexpr bool QUAD = MODE == MODE_PW;
+    constexpr bool SWAP = PREC == PREC_FP32 && MT >= 3;
+    const int b_last = b_end - 1;
+    auto quad_rx = [&](int b0) {
+        const int bq = min(b0 + g, b_last);
+        double Fr = A.Frx[bq];
+        if (A.clamp) Fr = fmin(Fr, A.sqrt_f[bq] * clamp_z);  // das_pw.m:374-377
+        return pack_aperture(make_aperture(x, xp, zp, Fr, A, A.support_rx));
+    };
+    ApWord q_rx{0u, 0.f, 0.f};  // this lane's bin of the current four
+
     for (int b = b_begin; b < b_end; ++b) {
         const int n = A.n_lb + b;
+        const int qb = (b - b_begin) & 3;
+        if (QUAD && qb == 0) q_rx = quad_rx(b);
+        double Fr = QUAD ? 0.0 : A.Frx[b];
+        if (!QUAD && A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_z);  // das_pw.m:374-377
+        const Aperture ap = QUAD ? unpack_aperture(row_aperture<SWAP>(q_rx, qb, c))
+                                 : make_aperture(x, xp, zp, Fr, A, A.support_rx);
         if (CHUNKED && !pipe) {
             __syncthreads();
             if constexpr (PREC == PREC_FP32) {
@@ -790,10 +865,6 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             }
             __syncthreads();
         }
-
-        double Fr = A.Frx[b];
-        if (A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_z);  // das_pw.m:374-377
-        const Aperture ap = make_aperture(x, xp, zp, Fr, A, A.support_rx);
 
         // elements [u_lo, u_hi] meeting the support of some voxel of this wave
         // (wave-uniform): element steps outside carry zero window for all 16 voxels

@@ -230,12 +230,18 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) __attribute__((amdgpu_waves_per_eu(
                     const uint32_t Qv = q_rx[cur][qoff[s] + 16 * mt];
                     const float ta = mt == 0 ? R0[s] : fmaf((float)mt, R1[s], R0[s]);
                     const float tb = mt == 0 ? L0[s] : fmaf((float)mt, L1[s], L0[s]);
+                    // max(ta, tb) as a signed-integer max of the bit patterns: fmaxf would
+                    // first canonicalize both operands (two more instructions per weight),
+                    // since they come from another basic block.  One of the two is >= 0 (each
+                    // side's term is non-negative on its own side), and on non-negative floats
+                    // the integer order is the float order, so the result is max(ta, tb)
+                    const float tm = __int_as_float(max(__float_as_int(ta), __float_as_int(tb)));
                     float w;
                     if (WCR == WC_COS_RF) {
-                        const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(fmaxf(ta, tb), 0.f, 0.25f));
+                        const float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fmed3f(tm, 0.f, 0.25f));
                         w = c * c;
                     } else {
-                        w = window_eval<WCR>(fmaxf(ta, tb), A.wrx);
+                        w = window_eval<WCR>(tm, A.wrx);
                     }
                     if (!FULL) w = (e >= Klo[s] && e <= Khi[s]) ? w : 0.f;
                     const float rv = rev_of(n * Qv);

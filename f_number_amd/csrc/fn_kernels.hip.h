@@ -807,9 +807,10 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
     // computes the aperture of bin b0 + g once per four bins and each bin takes its own
     // from group (b - b0) & 3: bitwise the same apertures.  The FP32 instances of three
     // or more tiles move it by lane swaps, the others by a lane permute (each measured
-    // the faster for them).  (The synthetic-aperture instances have no registers to
-    // spare for it.)
-    constexpr bool QUAD = MODE == MODE_PW;
+    // the faster for them).  The synthetic aperture's FP32 instances do the same for
+    // their transmit apertures; its 16-bit forms measured slower with it (and the split
+    // ones and the windowed normalization have no registers to spare for it).
+    constexpr bool QUAD = MODE == MODE_PW || (MODE == MODE_SA && !NW && PREC == PREC_FP32);
     constexpr bool SWAP = PREC == PREC_FP32 && MT >= 3;
     const int b_last = b_end - 1;
     auto quad_rx = [&](int b0) {
@@ -818,12 +819,19 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
         if (A.clamp) Fr = fmin(Fr, A.sqrt_f[bq] * clamp_z);  // das_pw.m:374-377
         return pack_aperture(make_aperture(x, xp, zp, Fr, A, A.support_rx));
     };
-    ApWord q_rx{0u, 0.f, 0.f};  // this lane's bin of the current four
+    auto quad_tx = [&](int b0) {  // das_sa.m:384-400
+        const int bq = min(b0 + g, b_last);
+        return pack_aperture(make_aperture(x, xp, zp, A.Ftx[bq], A, A.support_tx));
+    };
+    ApWord q_rx{0u, 0.f, 0.f}, q_tx{0u, 0.f, 0.f};  // this lane's bin of the current four
 
     for (int b = b_begin; b < b_end; ++b) {
         const int n = A.n_lb + b;
         const int qb = (b - b_begin) & 3;
-        if (QUAD && qb == 0) q_rx = quad_rx(b);
+        if (QUAD && qb == 0) {
+            q_rx = quad_rx(b);
+            if (MODE == MODE_SA) q_tx = quad_tx(b);
+        }
         double Fr = QUAD ? 0.0 : A.Frx[b];
         if (!QUAD && A.clamp) Fr = fmin(Fr, A.sqrt_f[b] * clamp_z);  // das_pw.m:374-377
         const Aperture ap = QUAD ? unpack_aperture(row_aperture<SWAP>(q_rx, qb, c))
@@ -1178,8 +1186,8 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
             if (NW) nw_row += ap.valid ? 1 : 0;
         } else {
             // transmit aperture and apodization (das_sa.m:384-400, 438-455)
-            const double Ft = A.Ftx[b];
-            const Aperture at = make_aperture(x, xp, zp, Ft, A, A.support_tx);
+            const Aperture at = QUAD ? unpack_aperture(row_aperture<SWAP>(q_tx, qb, c))
+                                     : make_aperture(x, xp, zp, A.Ftx[b], A, A.support_tx);
             float wtx[MT][4];
             float tsum = 0.f;
             // WC_COS_RF: cos^2 of the quarter-period argument max(k r1 + r0, k l1 + l0) for

@@ -126,10 +126,10 @@ hipError_t FN_LBL_LAUNCHER(INST_KI)(int wcr, dim3 grid, hipStream_t s, const Bea
 #if INST_MODE == 0 && !FN_WIDE
 // ensemble variant: frames x angles as transmit rows, up to four tiles per group
 #define FN_ENS_LAUNCHER(k, p) FN_CAT6(launch_beamform_ens, _k, k, _p, p, )
-template <int MT, int WCR>
+template <int MT, int WCR, bool STK>
 static hipError_t launch_ens_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
-    auto kern = beamform_kernel<0, MT, INST_KI, WCR, WC_RANGE, INST_PREC, false, true>;
+    auto kern = beamform_kernel<0, MT, INST_KI, WCR, WC_RANGE, INST_PREC, false, true, false, false, STK>;
     static size_t lds_done = 0;
     lds = dyn_lds(lds);
     if (hipError_t e = set_lds_once(kern, lds, lds_done)) return e;
@@ -137,35 +137,51 @@ static hipError_t launch_ens_one(dim3 grid, size_t lds, hipStream_t s, const Bea
     return hipGetLastError();
 }
 
-template <int MT>
+template <int MT, bool STK>
 static hipError_t launch_ens_mt(int wcr, dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
 {
     switch (wcr) {
-    case WC_RANGE: return launch_ens_one<MT, WC_RANGE>(grid, lds, s, A);
-    case WC_LINEAR: return launch_ens_one<MT, WC_LINEAR>(grid, lds, s, A);
-    case WC_COS_ALL: return launch_ens_one<MT, WC_COS_ALL>(grid, lds, s, A);
-    case WC_COS_RF: return launch_ens_one<MT, WC_COS_RF>(grid, lds, s, A);
-    default: return launch_ens_one<MT, WC_COS>(grid, lds, s, A);
+    case WC_RANGE: return launch_ens_one<MT, WC_RANGE, STK>(grid, lds, s, A);
+    case WC_LINEAR: return launch_ens_one<MT, WC_LINEAR, STK>(grid, lds, s, A);
+    case WC_COS_ALL: return launch_ens_one<MT, WC_COS_ALL, STK>(grid, lds, s, A);
+    case WC_COS_RF: return launch_ens_one<MT, WC_COS_RF, STK>(grid, lds, s, A);
+    default: return launch_ens_one<MT, WC_COS, STK>(grid, lds, s, A);
     }
 }
 
+// stack != 0: a row stack (rows of several acquisitions, each its own transmit phase)
 hipError_t FN_ENS_LAUNCHER(INST_KI, INST_PREC)(int mt, int wcr, dim3 grid, size_t lds, hipStream_t s,
-                                                const BeamArgs &A)
+                                                const BeamArgs &A, int stack)
 {
+    if (stack) {
+        // groups of at most the default tile count (the row stacks' tiling)
+        switch (mt) {
+        case 1: return launch_ens_mt<1, true>(wcr, grid, lds, s, A);
+        case 2: return launch_ens_mt<2, true>(wcr, grid, lds, s, A);
+        case 3: return launch_ens_mt<3, true>(wcr, grid, lds, s, A);
+#if INST_PREC == 0 || INST_PREC == 3
+        case 4: return launch_ens_mt<4, true>(wcr, grid, lds, s, A);
+#endif
+#if INST_PREC == 3 && INST_KI == 32
+        case 5: return launch_ens_mt<5, true>(wcr, grid, lds, s, A);
+#endif
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (mt) {
-    case 1: return launch_ens_mt<1>(wcr, grid, lds, s, A);
-    case 2: return launch_ens_mt<2>(wcr, grid, lds, s, A);
-    case 3: return launch_ens_mt<3>(wcr, grid, lds, s, A);
+    case 1: return launch_ens_mt<1, false>(wcr, grid, lds, s, A);
+    case 2: return launch_ens_mt<2, false>(wcr, grid, lds, s, A);
+    case 3: return launch_ens_mt<3, false>(wcr, grid, lds, s, A);
 #if INST_PREC == 0 || INST_PREC == 3  // FP32, FP16; the split forms stop at 3 tiles (register file)
-    case 4: return launch_ens_mt<4>(wcr, grid, lds, s, A);
+    case 4: return launch_ens_mt<4, false>(wcr, grid, lds, s, A);
 #endif
 #if INST_PREC == 3 && INST_KI == 32
     // fp16 at 128 elements: wider groups, one wave per SIMD (offered to the autotuner,
     // not the default; at 64 elements 7 and 8 tiles would spill)
-    case 5: return launch_ens_mt<5>(wcr, grid, lds, s, A);
-    case 6: return launch_ens_mt<6>(wcr, grid, lds, s, A);
-    case 7: return launch_ens_mt<7>(wcr, grid, lds, s, A);
-    case 8: return launch_ens_mt<8>(wcr, grid, lds, s, A);
+    case 5: return launch_ens_mt<5, false>(wcr, grid, lds, s, A);
+    case 6: return launch_ens_mt<6, false>(wcr, grid, lds, s, A);
+    case 7: return launch_ens_mt<7, false>(wcr, grid, lds, s, A);
+    case 8: return launch_ens_mt<8, false>(wcr, grid, lds, s, A);
 #endif
     default: return hipErrorInvalidValue;
     }

@@ -36,15 +36,16 @@ FN_DECL_LAUNCHERS(1, 48)
 FN_DECL_LAUNCHERS(1, 64)
 #undef FN_DECL_LAUNCHERS
 #define FN_DECL_ENS(k)                                                                              \
-    hipError_t launch_beamform_ens_k##k##_p0(int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
-    hipError_t launch_beamform_ens_k##k##_p1(int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
-    hipError_t launch_beamform_ens_k##k##_p2(int, int, dim3, size_t, hipStream_t, const BeamArgs &); \
-    hipError_t launch_beamform_ens_k##k##_p3(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+    hipError_t launch_beamform_ens_k##k##_p0(int, int, dim3, size_t, hipStream_t, const BeamArgs &, int); \
+    hipError_t launch_beamform_ens_k##k##_p1(int, int, dim3, size_t, hipStream_t, const BeamArgs &, int); \
+    hipError_t launch_beamform_ens_k##k##_p2(int, int, dim3, size_t, hipStream_t, const BeamArgs &, int); \
+    hipError_t launch_beamform_ens_k##k##_p3(int, int, dim3, size_t, hipStream_t, const BeamArgs &, int);
 FN_DECL_ENS(8)
 FN_DECL_ENS(16)
 FN_DECL_ENS(32)
 #undef FN_DECL_ENS
-typedef hipError_t (*EnsLauncher)(int, int, dim3, size_t, hipStream_t, const BeamArgs &);
+// the last argument: 1 for a row stack (per-row transmit phases), 0 for an ensemble
+typedef hipError_t (*EnsLauncher)(int, int, dim3, size_t, hipStream_t, const BeamArgs &, int);
 // [KI = 8, 16, 32][precision]
 static const EnsLauncher kEnsLaunchers[3][4] = {
     {launch_beamform_ens_k8_p0, launch_beamform_ens_k8_p1, launch_beamform_ens_k8_p2, launch_beamform_ens_k8_p3},
@@ -136,6 +137,7 @@ struct fn_plan {
     long n_vox = 0;
     int KI = 0, MT_all = 0, MT_tmpl = 0, n_groups = 1, bin_split = 1;
     int n_frames = 1;  // ensemble frames (transmit rows = frames x angles)
+    int ens_angles = 0;  // ensemble: angles per frame
     int wcr = 0, wct = 0;
     int nw = 0;                 // windowed normalization (CPU semantics): taper kernels, one bin chunk
     int lbl = 0;                // line scan: grid column i from transmit row i (groups = columns)
@@ -419,7 +421,10 @@ int configure_common(fn_plan *P, const Common &c, const fnh::Grid &g)
             return fail(FN_ERR_UNSUPPORTED, "normalizations.windowed (CPU semantics): at most 128 elements (got %d)",
                         c.n_el);
     }
-    P->MT_all = (c.n_tx + 15) / 16;
+    // operand rows: an ensemble's angles x frame slots (frames rounded up to a multiple
+    // of 4, angle-major), otherwise the transmit rows
+    const int op_rows = P->n_frames > 1 ? P->ens_angles * P->args.ens_slots : c.n_tx;
+    P->MT_all = (op_rows + 15) / 16;
     if (P->lbl) {
         // line scan: one group per grid column, staging the M-tile of its transmit row
         P->MT_tmpl = 1;
@@ -471,7 +476,7 @@ int configure_common(fn_plan *P, const Common &c, const fnh::Grid &g)
     fnk::BeamArgs &A = P->args;
     A.ctof = c.c0;
     A.dimage = nullptr;
-    A.n_x = c.n_x; A.n_z = c.n_z; A.n_el = c.n_el; A.n_tx = c.n_tx;
+    A.n_x = c.n_x; A.n_z = c.n_z; A.n_el = c.n_el; A.n_tx = op_rows;
     A.n_f = g.n_f; A.n_lb = g.n_lb; A.N = g.N;
     A.bin_chunk = chunk;
     A.tx_group = P->MT_tmpl;
@@ -482,8 +487,11 @@ int configure_common(fn_plan *P, const Common &c, const fnh::Grid &g)
 
     fnk::SpecArgs &S = P->sargs;
     S.n_t = c.n_t; S.n_ch = c.n_el * c.n_tx; S.N = g.N; S.n_lb = g.n_lb; S.n_f = g.n_f;
-    S.n_el = c.n_el; S.n_tx = c.n_tx; S.MT = P->MT_all; S.KI = P->KI;
+    S.n_el = c.n_el; S.n_tx = op_rows; S.MT = P->MT_all; S.KI = P->KI;
     S.row0 = 0; S.MT_out = P->MT_all; S.write_pad = 1;
+    S.ens_slots = P->args.ens_slots;
+    S.ens_frames = P->args.ens_frames;
+    S.ens_angles = P->n_frames > 1 ? P->ens_angles : 0;
 
     P->flops = 8.0 * g.n_f * (double)P->n_vox * c.n_el * (P->lbl ? 1 : c.n_tx);
     return FN_OK;
@@ -592,7 +600,7 @@ hipError_t launch_beam(const fn_plan *P, dim3 grid, const fnk::BeamArgs &A, hipS
     const int kidx = ki_index(P->KI);  // ensembles, line scans and windowed plans stop at KI = 32
     if (P->lbl) return fnk::kLblLaunchers[kidx](P->wcr, grid, s, A);
     if (P->nw) return fnk::kNwLaunchers[P->mode][kidx](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
-    if (P->n_frames > 1) return fnk::kEnsLaunchers[kidx][P->precision](P->MT_tmpl, P->wcr, grid, P->lds_beam, s, A);
+    if (P->n_frames > 1) return fnk::kEnsLaunchers[kidx][P->precision](P->MT_tmpl, P->wcr, grid, P->lds_beam, s, A, 0);
     return fnk::kBeamLaunchers[P->mode][kidx][P->precision](P->MT_tmpl, P->wcr, P->wct, grid, P->lds_beam, s, A);
 }
 
@@ -762,9 +770,13 @@ int configure_pw(fn_plan *P, const fn_pw_params *p, Common *c)
     P->args.support_rx = wsup;
     P->args.norm_rx = p->normalization != FN_NORMALIZATION_OFF;
     P->args.clamp = p->semantics == FN_SEMANTICS_CPU;
-    // transmit rows: the angles of every frame (row = frame * n_angles + angle)
+    // transmit rows: every angle of every frame.  An ensemble's operand rows are
+    // angle-major over frame slots (row = angle * slots + slot, slots = frames rounded up
+    // to a multiple of 4), so that the four rows a lane holds per M-tile share one angle
     const int n_rows = p->n_angles * P->n_frames;
-    P->args.rows_per_frame = P->n_frames > 1 ? p->n_angles : 0;
+    P->args.ens_frames = P->n_frames > 1 ? P->n_frames : 0;
+    P->args.ens_slots = P->n_frames > 1 ? (P->n_frames + 3) / 4 * 4 : 0;
+    P->ens_angles = p->n_angles;
     P->args.tx_kind = p->virtual_sources ? fnk::TX_VIRTUAL : fnk::TX_PLANE;
     *c = Common{p->positions_x, p->positions_z, p->n_x, p->n_z, p->n_samples, p->n_elements, n_rows,
                 p->f_s, p->element_width, p->element_pitch, p->c_0, p->index_t0,
@@ -781,14 +793,14 @@ int pw_plan(const fn_pw_params *p, fn_plan **out)
     if (!P) return fail(FN_ERR_OUT_OF_MEMORY, "host allocation");
     Common c{};
     if ((s = configure_pw(P, p, &c))) { destroy(P); return s; }
-    const int n_rows = c.n_tx;
     if ((s = create_common(P, c, g, p->f_number_values, nullptr))) { destroy(P); return s; }
+    const int n_rows = P->args.n_tx;  // operand rows (ensembles: frame slots)
     // steering geometry in FP64: das_pw.m:323-327 (x_ref = sign(e_x) x_0), or the
     // virtual sources (x_vs, z_vs, |r_vs|)
     std::vector<double> txp(3 * (size_t)n_rows);
     const double M = (p->n_elements - 1) / 2.0;
     for (int row = 0; row < n_rows; ++row) {
-        const int a = row % p->n_angles;
+        const int a = P->n_frames > 1 ? row / P->args.ens_slots : row;  // angle-major ensemble rows
         if (p->virtual_sources) {
             const double xv = p->virtual_sources[2 * a], zv = p->virtual_sources[2 * a + 1];
             txp[3 * row] = xv;
@@ -1896,7 +1908,8 @@ static int batch_run_stack(fn_batch *b, const RowStack &R, const void *const *d_
     A.n_tx = R.rows;
     A.tx_group = R.MT_tmpl;
     A.bin_chunk = R.chunk;
-    A.rows_per_frame = 0;
+    A.ens_frames = 0;
+    A.ens_slots = 0;
     A.row_frame = R.d_row_frame;
     A.frame_image = R.d_frame_image;
     A.image = nullptr;
@@ -1906,7 +1919,7 @@ static int batch_run_stack(fn_batch *b, const RowStack &R, const void *const *d_
     A.n_slots = 1;
     A.slot_z = R.n_groups;
     const dim3 grid((unsigned)beam_tiles(P0), (unsigned)R.bin_split, (unsigned)R.n_groups);
-    hipError_t e = fnk::kEnsLaunchers[ki_index(P0->KI)][P0->precision](R.MT_tmpl, P0->wcr, grid, R.lds_beam, s, A);
+    hipError_t e = fnk::kEnsLaunchers[ki_index(P0->KI)][P0->precision](R.MT_tmpl, P0->wcr, grid, R.lds_beam, s, A, 1);
     if (e != hipSuccess) return hip_fail(e, "beamform_kernel (row stack)");
     return FN_OK;
 }

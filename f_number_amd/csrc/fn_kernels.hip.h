@@ -93,8 +93,12 @@ struct BeamArgs {
     float2 *partial;
     int slot_z, n_slots;
     double M, pitch, width, c0, f_s, t0;
-    int rows_per_frame;     // ensemble: transmit rows per frame (angles); images [frame][n_vox]
-    const int *row_frame;   // row stack: frame of every transmit row (rows_per_frame unused)
+    // ensemble: F = ens_frames frames in ens_slots = F rounded up to a multiple of 4
+    // frame slots; its transmit rows are angle-major, row = angle ens_slots + slot, so the
+    // four rows a lane holds per M-tile share one angle and one transmit phase (rows of
+    // slots >= F have zero spectra and no image); images [frame][n_vox]
+    int ens_slots, ens_frames;
+    const int *row_frame;   // row stack: frame of every transmit row (ens_slots 0)
     float2 *const *frame_image;  // row stack: image of every frame
     int tx_kind;            // TX_PLANE: txp = (e_x, e_z, x_ref); TX_VIRTUAL: txp = (x_vs, z_vs, |r_vs|)
     double ctof;            // speed of sound of the time of flight (c0 unless set)
@@ -127,6 +131,10 @@ struct SpecArgs {
     // tiles (a row stack of several acquisitions); write_pad also writes the zero rows
     // past n_tx of the last tile (a plan of its own)
     int row0, MT_out, write_pad;
+    // ensemble: output row R = angle S + slot reads RF row slot P + angle when slot < F
+    // (F frames of P angles, S frame slots; a row of a slot >= F is zero); ens_slots 0:
+    // output row = RF row
+    int ens_slots, ens_frames, ens_angles;
 };
 
 __device__ __forceinline__ uint32_t frac32(double q)
@@ -405,15 +413,30 @@ __global__ __launch_bounds__(256) void spectra_kernel(const T *__restrict__ rf, 
     // per-i offsets from the thread's base are wave-uniform)
     const int tl_own = tid & 63, cl_own = tid >> 6;
     const int a_own = 16 * A + cl_own;
-    const T *src = rf + ((size_t)a_own * S.n_el + 4 * K4) * S.n_t + tl_own;
+    // RF row of output rows a_own + 4 u (ensembles: angle-major output rows over frame
+    // slots, the slots past the last frame read nothing)
+    size_t roff[4];
+    bool rok[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int R = a_own + 4 * u;
+        int in = R;
+        rok[u] = R < S.n_tx;
+        if (S.ens_slots > 0) {
+            const int slot = R % S.ens_slots;
+            in = slot * S.ens_angles + R / S.ens_slots;
+            rok[u] = rok[u] && slot < S.ens_frames;
+        }
+        roff[u] = rok[u] ? ((size_t)in * S.n_el + 4 * K4) * S.n_t + tl_own : 0;
+    }
     float pre[16];
     auto load_chunk = [&](int t0) {
         const bool tok = t0 + tl_own < S.n_t;
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const int ra = 4 * (i & 3), ke = i >> 2;
-            const bool ok = tok && a_own + ra < S.n_tx && 4 * K4 + ke < S.n_el;
-            pre[i] = ok ? (float)src[((size_t)ra * S.n_el + ke) * S.n_t + t0] : 0.f;
+            const int ke = i >> 2;
+            const bool ok = tok && rok[i & 3] && 4 * K4 + ke < S.n_el;
+            pre[i] = ok ? (float)rf[roff[i & 3] + (size_t)ke * S.n_t + t0] : 0.f;
         }
     };
     load_chunk(0);
@@ -614,19 +637,21 @@ struct PersistWavesPerEU {
 // TAN (plane waves, FP32): also accumulates the tangent dI/dc_tof.  Geometric
 // delays scale as 1/c, so d/dc e^{j w tau} = j w (-(tau_tx,g + tau_rx,g) / c) e^{j w tau}:
 // a second contraction with B' = B tau_rx,g and the row factor tau_tx,g.
-// ENS (plane waves): the transmit rows are (frame, angle) pairs of an ensemble
-// sharing one geometry; every row keeps its own image accumulator over the
-// bins and is added to its frame's image at the end (frame = row / rows_per_frame,
-// or row_frame[row] for a row stack of acquisitions with their own images).
+// ENS (plane waves): the transmit rows are (angle, frame slot) pairs of an ensemble
+// sharing one geometry, angle-major; every row keeps its own image accumulator over
+// the bins and is added to its frame's image at the end (frame = row % ens_slots when
+// below ens_frames, or row_frame[row] for a row stack of acquisitions with their own
+// images: a stack's rows are distinct angles, each row with its own transmit phase).
 // NW: windowed normalization (FP32, one bin chunk per block so that every lane
 // counts its voxel's valid rows from the first bin).
 // One workgroup's share of a plan: voxel tile bx (64 voxels), bin chunk by,
 // transmit group bz.  beamform_kernel runs it for its block index; the
 // persistent batch kernel for every work item it claims.
 template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false,
-          bool LBL = false, typename Args = BeamArgs>
+          bool LBL = false, bool STK = false, typename Args = BeamArgs>
 __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by, const int bz, const int tid)
 {
+    static_assert(!STK || ENS, "row stacks run the ensemble form");
     static_assert(!TAN || (MODE == MODE_PW && PREC == PREC_FP32), "tangent: plane waves, FP32");
     static_assert(!ENS || (MODE == MODE_PW && !TAN), "ensemble: plane waves");
     static_assert(!NW || (PREC == PREC_FP32 && !TAN && !ENS), "windowed normalization: FP32 forward");
@@ -1150,16 +1175,32 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                 } else {
                 float Sr = 0.f, Si = 0.f;  // sum over the rows of T Y, scaled once below
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
+                for (int mt = 0; mt < MT; ++mt) {
+                    // ensemble: the four rows of this lane in the tile share their angle
+                    // (angle-major rows over frame slots), so one transmit phase, scaled by
+                    // sc; a row stack's rows (STK) each have their own
+                    float sn0 = 0.f, cs0 = 0.f;
+                    constexpr bool shared = ENS && !STK;
+                    if (shared) {
+                        const float rv = rev_of((uint32_t)n * Qtx[mt][0]);
+                        sn0 = __builtin_amdgcn_sinf(rv) * sc;
+                        cs0 = __builtin_amdgcn_cosf(rv) * sc;
+                    }
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
+                        if (ENS) {
+                            if (!shared) {
+                                const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
+                                sn0 = __builtin_amdgcn_sinf(rv) * sc;
+                                cs0 = __builtin_amdgcn_cosf(rv) * sc;
+                            }
+                            Er[ENS ? mt : 0][r] = fmaf(cs0, accR[mt][r], fmaf(-sn0, accI[mt][r], Er[ENS ? mt : 0][r]));
+                            Ei[ENS ? mt : 0][r] = fmaf(cs0, accI[mt][r], fmaf(sn0, accR[mt][r], Ei[ENS ? mt : 0][r]));
+                            continue;
+                        }
                         const float rv = rev_of((uint32_t)n * Qtx[mt][r]);
                         const float sn = __builtin_amdgcn_sinf(rv), cs = __builtin_amdgcn_cosf(rv);
-                        if (ENS) {
-                            const float yr = accR[mt][r] * sc, yi = accI[mt][r] * sc;
-                            Er[ENS ? mt : 0][r] += cs * yr - sn * yi;
-                            Ei[ENS ? mt : 0][r] += cs * yi + sn * yr;
-                        } else {
+                        {
                             Sr = fmaf(cs, accR[mt][r], fmaf(-sn, accI[mt][r], Sr));
                             Si = fmaf(cs, accI[mt][r], fmaf(sn, accR[mt][r], Si));
                         }
@@ -1171,6 +1212,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
                             pi += cs * ui + sn * ur;
                         }
                     }
+                }
                 if (!ENS) {
                     Ir = fmaf(sc, Sr, Ir);
                     Ii = fmaf(sc, Si, Ii);
@@ -1259,7 +1301,8 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
         if (b == A.n_f - 1) w_last = asum;
     }
     if (ENS) {
-        // rows of this lane -> their frames; consecutive rows of one frame are summed first
+        // rows of this lane -> their frames; consecutive rows of one frame (row stacks) are
+        // summed first
         if (vok) {
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
@@ -1268,7 +1311,15 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * (mt0 + mt) + 4 * g + r;
-                    const int f = row < A.n_tx ? (A.row_frame ? A.row_frame[row] : row / A.rows_per_frame) : -1;
+                    int f = -1;
+                    if (row < A.n_tx) {
+                        if (STK) {
+                            f = A.row_frame[row];
+                        } else {
+                            const int slot = row % A.ens_slots;
+                            f = slot < A.ens_frames ? slot : -1;
+                        }
+                    }
                     if (f != f_run) {
                         if (f_run >= 0) {
                             float2 *img = A.frame_image ? A.frame_image[f_run] : A.image + (size_t)f_run * n_vox;
@@ -1320,7 +1371,7 @@ __device__ __forceinline__ void beamform_tile(Args A, const int bx, const int by
 }
 
 template <int MODE, int MT, int KI, int WCR, int WCT, int PREC, bool TAN = false, bool ENS = false, bool NW = false,
-          bool LBL = false>
+          bool LBL = false, bool STK = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BeamWavesPerEU<MT, KI, WCR, PREC, ENS>::value)))
 void beamform_kernel(BeamArgs A)
 {
@@ -1329,8 +1380,8 @@ void beamform_kernel(BeamArgs A)
     // reference form leaves the compiler the registers it schedules best with
     using Args =
         typename std::conditional<(MT * KI <= 64 || PREC != PREC_FP32), BeamArgs, const BeamArgs &>::type;
-    beamform_tile<MODE, MT, KI, WCR, WCT, PREC, TAN, ENS, NW, LBL, Args>(A, blockIdx.x, blockIdx.y, blockIdx.z,
-                                                                      threadIdx.x);
+    beamform_tile<MODE, MT, KI, WCR, WCT, PREC, TAN, ENS, NW, LBL, STK, Args>(A, blockIdx.x, blockIdx.y, blockIdx.z,
+                                                                           threadIdx.x);
 }
 
 // Persistent batch form: the work items (job, voxel tile, bin chunk, transmit
@@ -1356,7 +1407,7 @@ void beamform_persistent_kernel(const BeamArgs *__restrict__ jobs, const int4 *_
         // recomputed per tile rather than hoisted out of the loop and held across tiles
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        beamform_tile<MODE, MT, KI, WCR, WCT, PREC, false, false, false, false, const BeamArgs &>(
+        beamform_tile<MODE, MT, KI, WCR, WCT, PREC, false, false, false, false, false, const BeamArgs &>(
             A, __builtin_amdgcn_readfirstlane(w.y),
                                                     __builtin_amdgcn_readfirstlane(w.z),
                                                     __builtin_amdgcn_readfirstlane(w.w), tid);

@@ -124,7 +124,8 @@ hipError_t FN_LBL_LAUNCHER(INST_KI)(int wcr, dim3 grid, hipStream_t s, const Bea
 #endif
 
 #if INST_MODE == 0 && !FN_WIDE
-// ensemble variant: frames x angles as transmit rows, up to four tiles per group
+// ensemble variant: frames x angles as transmit rows, up to four tiles per group (fp16
+// at 128 elements up to eight)
 #define FN_ENS_LAUNCHER(k, p) FN_CAT6(launch_beamform_ens, _k, k, _p, p, )
 template <int MT, int WCR, bool STK>
 static hipError_t launch_ens_one(dim3 grid, size_t lds, hipStream_t s, const BeamArgs &A)
@@ -159,9 +160,7 @@ hipError_t FN_ENS_LAUNCHER(INST_KI, INST_PREC)(int mt, int wcr, dim3 grid, size_
         case 1: return launch_ens_mt<1, true>(wcr, grid, lds, s, A);
         case 2: return launch_ens_mt<2, true>(wcr, grid, lds, s, A);
         case 3: return launch_ens_mt<3, true>(wcr, grid, lds, s, A);
-#if INST_PREC == 0 || INST_PREC == 3
         case 4: return launch_ens_mt<4, true>(wcr, grid, lds, s, A);
-#endif
 #if INST_PREC == 3 && INST_KI == 32
         case 5: return launch_ens_mt<5, true>(wcr, grid, lds, s, A);
 #endif
@@ -172,9 +171,7 @@ hipError_t FN_ENS_LAUNCHER(INST_KI, INST_PREC)(int mt, int wcr, dim3 grid, size_
     case 1: return launch_ens_mt<1, false>(wcr, grid, lds, s, A);
     case 2: return launch_ens_mt<2, false>(wcr, grid, lds, s, A);
     case 3: return launch_ens_mt<3, false>(wcr, grid, lds, s, A);
-#if INST_PREC == 0 || INST_PREC == 3  // FP32, FP16; the split forms stop at 3 tiles (register file)
     case 4: return launch_ens_mt<4, false>(wcr, grid, lds, s, A);
-#endif
 #if INST_PREC == 3 && INST_KI == 32
     // fp16 at 128 elements: wider groups, one wave per SIMD (offered to the autotuner,
     // not the default; at 64 elements 7 and 8 tiles would spill)

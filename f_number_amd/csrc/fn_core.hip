@@ -339,13 +339,13 @@ void use_cos_all(fn_plan *P)
 int wcr_general(int wcr) { return (wcr == fnk::WC_COS_ALL || wcr == fnk::WC_COS_RF) ? fnk::WC_COS : wcr; }
 
 // default transmit tiles per plane-wave workgroup that fit the register file
-// without spills: ensembles hold per-row image accumulators, the split 16-bit
-// forms their residual operands on top of that; fp16 at 128 elements has room for 5
+// without spills: ensembles (and row stacks) hold per-row image accumulators, so 4;
+// fp16 at 128 elements has room for 5
 int tile_cap(int n_frames, int precision, int KI)
 {
     if (n_frames <= 1) return 5;
     if (precision == FN_PRECISION_FP16 && KI == 32) return 5;
-    return (precision == FN_PRECISION_BF16X3 || precision == FN_PRECISION_FP16X3) ? 3 : 4;
+    return 4;
 }
 
 // element steps per lane: 4 KI >= n_el elements in four lane groups
@@ -431,7 +431,7 @@ int configure_common(fn_plan *P, const Common &c, const fnh::Grid &g)
         P->n_groups = c.n_x;
     } else if (P->mode == 0) {
         // at most 5 transmit tiles per group keep the LDS tile within 80 KiB
-        // (ensembles: 3-5, for the per-row image accumulators)
+        // (ensembles: 4-5, for the per-row image accumulators)
         const int max_tiles = P->nw ? 2 : std::min(tile_cap(P->n_frames, c.precision, P->KI), pw_mt_wide(P->KI));
         const int groups = (P->MT_all + max_tiles - 1) / max_tiles;
         P->MT_tmpl = (P->MT_all + groups - 1) / groups;

@@ -92,27 +92,29 @@ def test_ensemble_matches_oracle(fa, oracle, torch, precision, tol):
     assert max(errs) <= tol
 
 
-def test_fp16_ensemble_tile_groups_agree(fa, torch):
-    """fp16 ensembles at 128 elements take groups of up to 8 transmit tiles (one wave
-    per SIMD); every group size must give the image of the default grouping."""
-    ws, rf = _frames(30, 5)  # 150 rows = 10 tiles of 16
+@pytest.mark.parametrize("precision,max_mt", [("fp16", 8), ("fp32", 4), ("bf16x3", 4), ("fp16x3", 4)])
+def test_ensemble_tile_groups_agree(fa, torch, precision, max_mt):
+    """Ensembles take groups of up to 4 transmit tiles (fp16 at 128 elements up to 8,
+    one wave per SIMD from 6 on); every group size must give the image of the default
+    grouping.  30 frames fill 32 frame slots of the angle-major rows."""
+    ws, rf = _frames(30, 5)  # 5 angles x 32 slots = 160 operand rows = 10 tiles of 16
     w = ws[0]
-    plan = _plan(fa, w, precision="fp16", n_frames=30)
+    plan = _plan(fa, w, precision=precision, n_frames=30)
     shape = (30, w.positions_x.size, w.positions_z.size)
     ref = torch.empty(shape, dtype=torch.complex64, device="cuda")
     rf_d = torch.from_numpy(rf).cuda()
     plan.execute(rf_d, ref)
     split = plan.info()["bin_split"]
-    for mt in range(1, 9):
+    for mt in range(1, max_mt + 1):
         plan.set_tiling(mt, split)
         img = torch.empty_like(ref)
         plan.execute(rf_d, img)
         torch.cuda.synchronize()
         err = (torch.linalg.vector_norm(img - ref) / torch.linalg.vector_norm(ref)).item()
-        print(f"case fp16 ensemble m_tiles {mt}: rel_l2 vs default grouping {err:.2e}", flush=True)
+        print(f"case {precision} ensemble m_tiles {mt}: rel_l2 vs default grouping {err:.2e}", flush=True)
         assert err < 1e-6
     with pytest.raises(fa.FNumberError):
-        plan.set_tiling(9, split)
+        plan.set_tiling(max_mt + 1, split)
     plan.close()
 
 

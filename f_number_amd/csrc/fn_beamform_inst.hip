@@ -172,9 +172,13 @@ hipError_t FN_ENS_LAUNCHER(INST_KI, INST_PREC)(int mt, int wcr, dim3 grid, size_
     case 2: return launch_ens_mt<2, false>(wcr, grid, lds, s, A);
     case 3: return launch_ens_mt<3, false>(wcr, grid, lds, s, A);
     case 4: return launch_ens_mt<4, false>(wcr, grid, lds, s, A);
+#if INST_PREC == 0 && INST_KI == 32
+    // FP32 at 128 elements: 5 tiles (the default)
+    case 5: return launch_ens_mt<5, false>(wcr, grid, lds, s, A);
+#endif
 #if INST_PREC == 3 && INST_KI == 32
-    // fp16 at 128 elements: wider groups, one wave per SIMD (offered to the autotuner,
-    // not the default; at 64 elements 7 and 8 tiles would spill)
+    // fp16 at 128 elements: 5 tiles (the default), and 6-8 at one wave per SIMD
+    // (offered to the autotuner; at 64 elements 7 and 8 tiles would spill)
     case 5: return launch_ens_mt<5, false>(wcr, grid, lds, s, A);
     case 6: return launch_ens_mt<6, false>(wcr, grid, lds, s, A);
     case 7: return launch_ens_mt<7, false>(wcr, grid, lds, s, A);

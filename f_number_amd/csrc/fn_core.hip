@@ -338,15 +338,18 @@ void use_cos_all(fn_plan *P)
 
 int wcr_general(int wcr) { return (wcr == fnk::WC_COS_ALL || wcr == fnk::WC_COS_RF) ? fnk::WC_COS : wcr; }
 
-// default transmit tiles per plane-wave workgroup that fit the register file
-// without spills: ensembles (and row stacks) hold per-row image accumulators, so 4;
-// fp16 at 128 elements has room for 5
+// default transmit tiles per plane-wave workgroup that fit the register file:
+// ensembles hold per-row image accumulators, so 4; FP32 and fp16 at 128 elements 5
+// (FP32 with one 8-byte reload per bin from scratch in the full-array cosine class)
 int tile_cap(int n_frames, int precision, int KI)
 {
     if (n_frames <= 1) return 5;
-    if (precision == FN_PRECISION_FP16 && KI == 32) return 5;
+    if ((precision == FN_PRECISION_FP16 || precision == FN_PRECISION_FP32) && KI == 32) return 5;
     return 4;
 }
+
+// transmit tiles per group of a row stack (its own kernel instances, per-row phases)
+int stack_tile_cap(int precision, int KI) { return (precision == FN_PRECISION_FP16 && KI == 32) ? 5 : 4; }
 
 // element steps per lane: 4 KI >= n_el elements in four lane groups
 int ki_of(int n_el)
@@ -1836,7 +1839,7 @@ static int batch_plan_stacks(fn_batch *b, float *const *d_images, std::vector<ch
         }
         const fn_plan *P0 = b->plans[i];
         r.MT_all = (r.rows + 15) / 16;
-        const int max_tiles = tile_cap(2, P0->precision, P0->KI);  // the ensemble kernel's register budget
+        const int max_tiles = stack_tile_cap(P0->precision, P0->KI);  // the stack kernel's register budget
         const int groups = (r.MT_all + max_tiles - 1) / max_tiles;
         r.MT_tmpl = (r.MT_all + groups - 1) / groups;
         r.n_groups = (r.MT_all + r.MT_tmpl - 1) / r.MT_tmpl;

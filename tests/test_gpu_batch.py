@@ -124,9 +124,9 @@ def test_ensemble_matches_oracle(fa, oracle, torch, precision, tol):
     assert max(errs) <= tol
 
 
-@pytest.mark.parametrize("precision,max_mt", [("fp16", 8), ("fp32", 4), ("bf16x3", 4), ("fp16x3", 4)])
+@pytest.mark.parametrize("precision,max_mt", [("fp16", 8), ("fp32", 5), ("bf16x3", 4), ("fp16x3", 4)])
 def test_ensemble_tile_groups_agree(fa, torch, precision, max_mt):
-    """Ensembles take groups of up to 4 transmit tiles (fp16 at 128 elements up to 8,
+    """Ensembles take groups of up to 4 transmit tiles (FP32 at 128 elements up to 5, fp16 up to 8,
     one wave per SIMD from 6 on); every group size must give the image of the default
     grouping.  30 frames fill 32 frame slots of the angle-major rows."""
     ws, rf = _frames(30, 5)  # 5 angles x 32 slots = 160 operand rows = 10 tiles of 16

@@ -74,6 +74,41 @@ def test_ensemble_frames_match_single_frame_plans(fa, torch, n_frames, n_ang, wi
     assert worst <= 1e-5
 
 
+@pytest.mark.parametrize("n_el,precision", [(64, "fp32"), (64, "fp16"), (40, "bf16x3"), (24, "fp32")])
+def test_small_array_ensembles_match_single_frame_plans(fa, torch, n_el, precision):
+    """Ensembles of arrays below 128 elements (16 and 8 element steps per lane, the 16-bit
+    forms with one dynamic LDS tile): 5 frames x 3 angles, 8 frame slots per angle."""
+    ws = []
+    for f in range(5):
+        w = synth_frame(f, n_el)
+        ws.append(w)
+    rf = np.stack([w.rf_host() for w in ws])
+    w = ws[0]
+    ens, single = _plan(fa, w, precision=precision, n_frames=5), _plan(fa, w, precision=precision)
+    nx, nz = w.positions_x.size, w.positions_z.size
+    imgs = torch.empty((5, nx, nz), dtype=torch.complex64, device="cuda")
+    ens.execute(torch.from_numpy(rf).cuda(), imgs)
+    one = torch.empty((nx, nz), dtype=torch.complex64, device="cuda")
+    worst = 0.0
+    for f in range(5):
+        single.execute(torch.from_numpy(rf[f]).cuda(), one)
+        torch.cuda.synchronize()
+        worst = max(worst, rel_l2(imgs[f].cpu().numpy(), one.cpu().numpy()))
+    print(f"case {precision} ensemble at {n_el} elements: worst frame rel_l2 vs single-frame plans {worst:.2e}",
+          flush=True)
+    ens.close()
+    single.close()
+    assert worst <= 1e-5
+
+
+def synth_frame(f, n_el):
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.doppler_frame(f * 40), 20, 24, n_angles=3, n_samples=1024, n_speckle=3000,
+                     n_elements=n_el)
+    return w
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
 def test_ensemble_of_virtual_sources_matches_single_frame_plans(fa, torch, precision):
     """An ensemble of diverging-wave frames (virtual sources instead of steering angles):

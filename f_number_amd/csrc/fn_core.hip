@@ -401,6 +401,14 @@ int upload(T **dst, const T *src, size_t n, const char *what)
     return FN_OK;
 }
 
+#ifndef FN_FEW_ROWS_BLOCKS
+#define FN_FEW_ROWS_BLOCKS 8192L
+#endif
+static bool few_rows_pw(const fn_plan *P)
+{
+    return P->mode == 0 && P->n_frames == 1 && !P->lbl && P->MT_tmpl <= 2;
+}
+
 // shared configuration of plan creation: tiling, launch shape and the scalar
 // arguments of the kernels; no device work and no allocation (the
 // no-allocation entry point runs it on a plan held on the stack)
@@ -460,8 +468,11 @@ int configure_common(fn_plan *P, const Common &c, const fnh::Grid &g)
         // line scans (vector-ALU bound, one column per group): about 8192 blocks below
         // 4096, measured best over bin splits 1..64 at 64-256 lines x 256-1024 depths
         if (blocks < 4096) split = (int)std::min<long>({(8192 + blocks - 1) / blocks, 16L, (long)std::max(1, g.n_f / 8)});
-    } else if (blocks < 2048 && !P->nw) {
-        split = (int)std::min<long>((2048 + blocks - 1) / blocks, std::max(1, g.n_f / 8));
+    } else if (!P->nw && blocks < (few_rows_pw(P) ? FN_FEW_ROWS_BLOCKS : 2048L)) {
+        // single-frame plane waves of one or two transmit tiles per group (vector-ALU
+        // bound element loops): about FN_FEW_ROWS_BLOCKS blocks, otherwise about 2048
+        const long target = few_rows_pw(P) ? FN_FEW_ROWS_BLOCKS : 2048L;
+        split = (int)std::min<long>((target + blocks - 1) / blocks, std::max(1, g.n_f / 8));
     } else if (P->mode == 0 && !P->nw && P->n_frames == 1) {
         // a group's band spectra beyond 32 MiB (many transmit tiles, a long band): the bins
         // are split so that the workgroups running together walk about 4 MiB of them (an

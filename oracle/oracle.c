@@ -30,6 +30,10 @@
  *                        windows, normalization on and off, mirror symmetry;
  *                        the OR_PLANT_* variants below are built by the tests
  *                        to show that these pins fail on such mistakes
+ *   memory / UB          the CPU oracle tests also run against a build with
+ *                        AddressSanitizer and UndefinedBehaviorSanitizer
+ *                        (OR_PLANT_OOB shows that this build reports a write
+ *                        past the end of an output)
  */
 #include "oracle.h"
 
@@ -218,6 +222,11 @@ int or_fnumber_values(const or_fnumber_t *F, const double *r, int n, double *out
         default: return -2;
         }
     }
+#ifdef OR_PLANT_OOB
+    /* planted mistake (the sanitizer test shows that its build reports it): one
+       element written past the end of out */
+    if (n > 0) out[n] = out[n - 1];
+#endif
     return 0;
 }
 

@@ -19,7 +19,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# FN_ORACLE_LIB loads another build of oracle.c (the sanitizer test's instrumented one)
+_LIB_PATH = os.environ.get("FN_ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle.so")
 
 WIN = {"boxcar": 0, "hann": 1, "tukey": 2, "cosine": 3, "triangular": 4}
 SEM = {"cpu": 0, "mex": 1}

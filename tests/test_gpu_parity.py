@@ -332,6 +332,45 @@ def test_virtual_source_synth_focuses_points(fa):
         assert abs(jx - 3) <= 1 and abs(jz - 3) <= 1, (xp, zp)
 
 
+@pytest.mark.parametrize("n_angles", [3, 40])
+@pytest.mark.parametrize("win,F", [("boxcar", "constant"), ("tukey", "angle_lb")])
+def test_short_bin_chunks_match_oracle(fa, oracle, win, F, n_angles):
+    """Bin chunks of one, two and three bins (bin_split up to n_f): the receive apertures
+    and support unions that lane groups compute once per four bins are clamped to the
+    chunk's last bin.  Boxcar / F = 1.5 (support test per element) and Tukey / angle_lb
+    (cut raised cosine) in FP32, one transmit tile (3 angles) and three (40 angles, lane
+    swaps), each split against the oracle and against split 1."""
+    import torch
+
+    from f_number_amd import synth
+
+    w = synth.scaled(synth.cpwc_small(), 20, 16, n_angles=n_angles, n_samples=1024, n_speckle=1500)
+    if n_angles > 11:
+        w.angles_deg = np.linspace(-10, 10, n_angles)
+    rf_h = w.rf_host()
+    Fo = fa.f_numbers.grating.angle_lb(60, 1.5) if F == "angle_lb" else fa.f_numbers.constant(1.5)
+    ref, info = oracle.das_pw(w.positions_x, w.positions_z, rf_h, w.f_s, w.angles, w.width, w.pitch, w.c_0,
+                              w.f_bounds, 0, (win, 0.5 if win == "tukey" else 0.0), _orc_F(oracle, Fo), "on")
+    n_f = info["n_ub"] - info["n_lb"] + 1
+    plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width, w.pitch,
+                     w.c_0, w.f_bounds, 0.0, window=_win(fa, win), F_number=Fo, output_scale="cpu")
+    rf = torch.from_numpy(rf_h).cuda()
+    shape = (w.positions_x.size, w.positions_z.size)
+    base = None
+    for split in (1, 2, 3, (n_f + 2) // 3, (n_f + 1) // 2, n_f):
+        plan.set_tiling(plan.info()["m_tiles"], split)
+        img_d = torch.empty(shape, dtype=torch.complex64, device="cuda")
+        plan.execute(rf, img_d)
+        torch.cuda.synchronize()
+        img = img_d.cpu().numpy().T
+        base = img if base is None else base
+        e, d, s = rel_l2(img, ref), peak_db(img, ref), rel_l2(img, base)
+        print(f"case {win}/{F} angles={n_angles} split={split} (n_f {n_f}): rel_l2={e:.3e} peak={d:.1f} dB "
+              f"vs split 1 {s:.1e}", flush=True)
+        assert e <= TOL_L2 and d <= TOL_DB and s <= 1e-6
+    plan.close()
+
+
 def test_split_images_are_reproducible_bitwise(fa):
     """Workgroups that share a voxel (8 bin chunks x 3 transmit groups here) write
     partial images that one kernel sums in a fixed order: two executions give the

@@ -371,6 +371,56 @@ def test_short_bin_chunks_match_oracle(fa, oracle, win, F, n_angles):
     plan.close()
 
 
+@pytest.mark.parametrize("form", ["sa_fp32", "pw_bf16x3", "pw_fp16"])
+def test_short_bin_chunks_other_forms(fa, oracle, form):
+    """The same bin chunks for the synthetic aperture's FP32 instance (receive and transmit
+    apertures once per four bins, lane swaps) and the 16-bit plane-wave forms (lane
+    permute): each split against the oracle (the form's own bar) and against split 1."""
+    import torch
+
+    from f_number_amd import synth
+
+    Fo = fa.f_numbers.grating.angle_lb(60, 1.5)
+    Fr = oracle.fnumber("angle_lb", 60, 1.5, 10)
+    win = fa.windows.tukey(0.5)
+    if form == "sa_fp32":
+        w = synth.scaled(synth.saft_full(), 12, 10, n_samples=1024, n_speckle=600, n_elements=32)
+        w.pitch = 4 * 0.3048e-3 / 1.0
+        w.width = 4 * 0.27e-3
+        rf_h = w.rf_host()
+        ref, info = oracle.das_sa(w.positions_x, w.positions_z, rf_h, w.f_s, w.width, w.pitch, w.c_0, w.f_bounds,
+                                  0, ("tukey", 0.5), ("tukey", 0.5), Fr, Fr, "on", "on")
+        plan = fa.SaPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.f_s, w.width, w.pitch, w.c_0,
+                         w.f_bounds, 0.0, window_tx=win, window_rx=win, F_number_tx=Fo, F_number_rx=Fo,
+                         output_scale="cpu")
+        tol = TOL_L2
+    else:
+        prec = form.split("_")[1]
+        w = synth.scaled(synth.cpwc_small(), 20, 16, n_angles=3, n_samples=1024, n_speckle=1500)
+        rf_h = w.rf_host()
+        ref, info = oracle.das_pw(w.positions_x, w.positions_z, rf_h, w.f_s, w.angles, w.width, w.pitch, w.c_0,
+                                  w.f_bounds, 0, ("tukey", 0.5), Fr, "on")
+        plan = fa.PwPlan(w.positions_x, w.positions_z, w.n_samples, w.n_elements, w.angles, w.f_s, w.width,
+                         w.pitch, w.c_0, w.f_bounds, 0.0, window=win, F_number=Fo, precision=prec,
+                         output_scale="cpu")
+        tol = {"bf16x3": 1e-4, "fp16": 1e-3}[prec]
+    n_f = info["n_ub"] - info["n_lb"] + 1
+    rf = torch.from_numpy(rf_h).cuda()
+    shape = (w.positions_x.size, w.positions_z.size)
+    base = None
+    for split in (1, 2, 3, (n_f + 2) // 3, (n_f + 1) // 2, n_f):
+        plan.set_tiling(plan.info()["m_tiles"], split)
+        img_d = torch.empty(shape, dtype=torch.complex64, device="cuda")
+        plan.execute(rf, img_d)
+        torch.cuda.synchronize()
+        img = img_d.cpu().numpy().T
+        base = img if base is None else base
+        e, s = rel_l2(img, ref), rel_l2(img, base)
+        print(f"case {form} split={split} (n_f {n_f}): rel_l2={e:.3e} vs split 1 {s:.1e}", flush=True)
+        assert e <= tol and s <= 1e-6
+    plan.close()
+
+
 def test_split_images_are_reproducible_bitwise(fa):
     """Workgroups that share a voxel (8 bin chunks x 3 transmit groups here) write
     partial images that one kernel sums in a fixed order: two executions give the

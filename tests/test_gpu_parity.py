@@ -339,7 +339,8 @@ def test_short_bin_chunks_match_oracle(fa, oracle, win, F, n_angles):
     and support unions that lane groups compute once per four bins are clamped to the
     chunk's last bin.  Boxcar / F = 1.5 (support test per element) and Tukey / angle_lb
     (cut raised cosine) in FP32, one transmit tile (3 angles) and three (40 angles, lane
-    swaps), each split against the oracle and against split 1."""
+    swaps), each split against the oracle and against split 1; the tangent instance's
+    image and dI/dc at each split against the forward image and split 1."""
     import torch
 
     from f_number_amd import synth
@@ -356,18 +357,22 @@ def test_short_bin_chunks_match_oracle(fa, oracle, win, F, n_angles):
                      w.c_0, w.f_bounds, 0.0, window=_win(fa, win), F_number=Fo, output_scale="cpu")
     rf = torch.from_numpy(rf_h).cuda()
     shape = (w.positions_x.size, w.positions_z.size)
-    base = None
+    base = dbase = None
     for split in (1, 2, 3, (n_f + 2) // 3, (n_f + 1) // 2, n_f):
         plan.set_tiling(plan.info()["m_tiles"], split)
-        img_d = torch.empty(shape, dtype=torch.complex64, device="cuda")
+        img_d, t_d, dt_d = (torch.empty(shape, dtype=torch.complex64, device="cuda") for _ in range(3))
         plan.execute(rf, img_d)
+        plan.execute_tangent(rf, t_d, dt_d)  # the tangent instance: image and dI/dc
         torch.cuda.synchronize()
-        img = img_d.cpu().numpy().T
+        img, t_img, dimg = img_d.cpu().numpy().T, t_d.cpu().numpy().T, dt_d.cpu().numpy().T
         base = img if base is None else base
+        dbase = dimg if dbase is None else dbase
         e, d, s = rel_l2(img, ref), peak_db(img, ref), rel_l2(img, base)
+        et, sd = rel_l2(t_img, img), rel_l2(dimg, dbase)
         print(f"case {win}/{F} angles={n_angles} split={split} (n_f {n_f}): rel_l2={e:.3e} peak={d:.1f} dB "
-              f"vs split 1 {s:.1e}", flush=True)
+              f"vs split 1 {s:.1e}; tangent image vs image {et:.1e}, dI/dc vs split 1 {sd:.1e}", flush=True)
         assert e <= TOL_L2 and d <= TOL_DB and s <= 1e-6
+        assert et <= 1e-5 and sd <= 1e-5
     plan.close()
 
 
